@@ -1,0 +1,199 @@
+/*
+ * afqmc_hip.h -- C-ABI of the MI355X walker step (plain pointers and sizes).
+ *
+ * The reference has no plugin or FFI seam: its driver calls C++ templates
+ * directly.  Each entry point below replaces one of those calls and keeps the
+ * reference's argument meaning and in-memory layouts (row-major, complex as
+ * interleaved double pairs, walker index LAST for G/vbias/X/vHS and FIRST for
+ * the walker set), so a caller holding the reference's arrays can switch by
+ * swapping the call.  All functions return an afqmc_status code; none throws.
+ * Each context owns one HIP stream; calls are ordered on it, and only calls
+ * that return host data synchronise.
+ *
+ * Reference paths are relative to the miniAFQMC source root.
+ */
+#ifndef AFQMC_HIP_H
+#define AFQMC_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AFQMC_OK = 0,
+  AFQMC_ERR_ARG = 1,         /* null pointer / bad enum */
+  AFQMC_ERR_SHAPE = 2,       /* inconsistent dimensions (reference: assert) */
+  AFQMC_ERR_ALLOC = 3,       /* host or device allocation failed */
+  AFQMC_ERR_HIP = 4,         /* HIP runtime error (message in afqmc_ctx_error) */
+  AFQMC_ERR_SINGULAR = 5,    /* exact zero pivot in an overlap matrix (reference: LAPACK info>0) */
+  AFQMC_ERR_NONFINITE = 6,   /* NaN/Inf in an input field buffer */
+  AFQMC_ERR_STATE = 7,       /* call out of order (e.g. no system uploaded) */
+  AFQMC_ERR_UNSUPPORTED = 8  /* valid request this build does not implement */
+} afqmc_status;
+
+/* Arithmetic of the Cholesky contractions and the propagator.  FP64 is kept
+ * for the overlap matrices, their LU/inverse, the log-overlap ratios, the
+ * orthogonalisation and every energy accumulation in all modes. */
+typedef enum {
+  AFQMC_FP64_CSR = 0,   /* reference algorithm: CSR csrmm for vbias/vHS/Vakbl, FP64 everywhere */
+  AFQMC_FP64_DENSE = 1, /* dense half-rotated / full Cholesky GEMMs on FP64 MFMA */
+  AFQMC_MIXED_FP32 = 2, /* complex FP32 MFMA contractions and propagator, FP32 accumulation */
+  AFQMC_MIXED_BF16 = 3  /* split-BF16 (hi+lo, 3 products) MFMA contractions and propagator, FP32 accumulation */
+} afqmc_mode;
+
+/* CSR in the layout of SparseMatrix<T> (src/Matrix/SparseMatrix.hpp:36-494):
+ * 0-based rowptr[nrows+1], col[nnz], val[2*nnz] (complex interleaved),
+ * rows sorted by (row, col). */
+typedef struct {
+  int nrows, ncols;
+  long long nnz;
+  const int* rowptr;
+  const int* col;
+  const double* val;
+} afqmc_csr_view;
+
+typedef struct afqmc_ctx afqmc_ctx;
+
+/* Buffers addressable with afqmc_copy(), in the reference's layouts
+ * (src/miniapps/miniafqmc_base.cpp:213-224). */
+typedef enum {
+  AFQMC_BUF_W = 0,       /* walkers        [nwalk][2][M][N] */
+  AFQMC_BUF_WDATA = 1,   /* walker data    [nwalk][8] */
+  AFQMC_BUF_G = 2,       /* compact G      [2*N*M][nwalk] */
+  AFQMC_BUF_VBIAS = 3,   /* force bias     [K][nwalk] */
+  AFQMC_BUF_X = 4,       /* shifted fields [K][nwalk] */
+  AFQMC_BUF_VHS = 5,     /* HS potential   [M*M][nwalk] */
+  AFQMC_BUF_HYBRIDW = 6  /* hybrid weight  [nwalk] */
+} afqmc_buffer;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces base::afqmc_sys::setup (src/AFQMC/afqmc_sys.hpp:61-93): sizes all
+ * device workspaces for M orbitals, N electrons per spin, K Cholesky vectors
+ * and nwalk walkers. */
+int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ctx** out);
+int afqmc_ctx_destroy(afqmc_ctx* ctx);
+/* message of the last failed call on this context ("" if none) */
+const char* afqmc_ctx_error(const afqmc_ctx* ctx);
+int afqmc_sync(afqmc_ctx* ctx);
+
+/* Replaces the data hand-off of afqmc::Initialize
+ * (src/Matrix/initialize_serial.hpp:35-228) followed by
+ * base::halfrotate_cholesky (src/AFQMC/rotate.hpp:65-169, called from
+ * src/miniapps/miniafqmc_base.cpp:181-185).  Inputs are what Initialize
+ * returns: trialwfn_alpha/beta = conj(Psi) [M][N], Propg1 [M][M], haj [2N][M],
+ * Spvn CSR [M*M][K] already scaled by sqrt(dt), and optionally Vakbl CSR
+ * [2NM][2NM] (NULL: the two-body energy uses the Cholesky form built from
+ * Spvn, which equals Vakbl's when Vakbl is built from the same vectors).
+ * The half rotation runs on the device; `cutoff` is rotate.hpp's cutoff for
+ * the CSR SpvnT used by AFQMC_FP64_CSR (dense modes keep every entry). */
+int afqmc_upload_system(afqmc_ctx* ctx, const double* trial_alpha, const double* trial_beta,
+                        const double* propg1, const double* haj, const afqmc_csr_view* spvn,
+                        const afqmc_csr_view* vakbl, double cutoff, double dt);
+/* The half-rotated, transposed Cholesky matrix SpvnT [K][2NM] built at
+ * upload (rotate.hpp:65-169 output).  Call with rowptr/col/val NULL to get
+ * nnz; AFQMC_FP64_CSR mode only. */
+int afqmc_get_spvnT(afqmc_ctx* ctx, long long* nnz, int* rowptr, int* col, double* val);
+
+/* Walker set: W = conj(trialwfn) per spin, W_data[n][1] = 1, rest 0
+ * (src/miniapps/miniafqmc_base.cpp:225-236). */
+int afqmc_init_walkers(afqmc_ctx* ctx);
+/* host <-> device copy of one buffer in the reference layout.  to_device=1
+ * uploads host -> device.  Buffers held in another layout or precision on
+ * the device are converted. */
+int afqmc_copy(afqmc_ctx* ctx, int buffer, int to_device, double* host);
+
+/* ---- phases of one substep (src/miniapps/miniafqmc_base.cpp:249-339) ---- */
+/* afqmc_sys::calculate_mixed_density_matrix(W, W_data, Gc, compact=true)
+ * (src/AFQMC/afqmc_sys.hpp:95-116): G and W_data[:,2..3] (overlaps). */
+int afqmc_mixed_density_matrix(afqmc_ctx* ctx);
+/* base::get_vbias(SpvnT, Gc, vbias, transposed=true) (src/AFQMC/vbias.hpp:45-83) */
+int afqmc_vbias(afqmc_ctx* ctx);
+/* X/hybrid-weight loop (src/miniapps/miniafqmc_base.cpp:281-289): x holds the
+ * real N(0,1) draws [K][nwalk] (NULL: use the fields staged for substep
+ * `substep` by afqmc_stage_fields).  X = x + i vbias; hybridW. */
+int afqmc_fields(afqmc_ctx* ctx, const double* x, long substep);
+/* base::get_vHS(Spvn, X, vHS) (src/AFQMC/vHS.hpp:43-56) */
+int afqmc_vhs(afqmc_ctx* ctx);
+/* afqmc_sys::propagate(W, Propg1, vHS) with base::apply_expM order 6
+ * (src/AFQMC/afqmc_sys.hpp:146-173, src/AFQMC/vHS.hpp:61-91) */
+int afqmc_propagate(afqmc_ctx* ctx);
+/* W_data[:,5] = W_data[:,4]; W_data[:,6..7] = W_data[:,2..3]
+ * (src/miniapps/miniafqmc_base.cpp:303-309) */
+int afqmc_save_overlaps(afqmc_ctx* ctx);
+/* afqmc_sys::calculate_overlaps (src/AFQMC/afqmc_sys.hpp:135-144) */
+int afqmc_calculate_overlaps(afqmc_ctx* ctx);
+/* weight / hybrid-energy / Eshift update (src/miniapps/miniafqmc_base.cpp:317-327) */
+int afqmc_weights(afqmc_ctx* ctx, double dt, double* eshift_inout);
+/* afqmc_sys::orthogonalize (src/AFQMC/afqmc_sys.hpp:175-207): Householder LQ
+ * with LAPACK's conventions (zgelqf + zunglq) on each W[w][s]. */
+int afqmc_orthogonalize(afqmc_ctx* ctx);
+/* afqmc_sys::calculate_energy (src/AFQMC/afqmc_sys.hpp:118-133,
+ * src/AFQMC/energy.hpp:63-102) on the current G: W_data[:,0] = E_L and the
+ * weighted average.  Call afqmc_mixed_density_matrix first, as the driver
+ * does (miniafqmc_base.cpp:342-343). */
+int afqmc_energy(afqmc_ctx* ctx, double* eav_out);
+
+/* ---- whole-loop entry points -------------------------------------------- */
+/* Device copy of the real N(0,1) draws of `nsub` substeps, x_all
+ * [nsub][K][nwalk]; afqmc_fields(ctx, NULL, s) and afqmc_run use them. */
+int afqmc_stage_fields(afqmc_ctx* ctx, const double* x_all, long nsub);
+/* One substep (miniafqmc_base.cpp:251-338): every phase above, with the
+ * orthogonalisation and overlap refresh when do_ortho != 0.  Draws come from
+ * the staged fields of `substep`.  Eshift is kept on the device; read it with
+ * afqmc_get_eshift. */
+int afqmc_substep(afqmc_ctx* ctx, long substep, int do_ortho, double dt);
+int afqmc_get_eshift(afqmc_ctx* ctx, double* eshift);
+int afqmc_set_eshift(afqmc_ctx* ctx, double eshift);
+/* The driver's step loop (miniafqmc_base.cpp:248-350) from substep index
+ * `first_substep` of the staged fields: nsteps blocks of nsubsteps substeps,
+ * orthogonalisation when the running substep counter (starting at
+ * first_substep) is a positive multiple of northo, and energy at the end of
+ * each block into eav_out[nsteps] (may be NULL to skip the energy). */
+int afqmc_run(afqmc_ctx* ctx, long first_substep, int nsteps, int nsubsteps, int northo, double dt,
+              double* eav_out);
+/* Capture one substep (with and without orthogonalisation) into HIP graphs
+ * and replay them from then on in afqmc_substep/afqmc_run (enable != 0). */
+int afqmc_use_graphs(afqmc_ctx* ctx, int enable);
+
+/* ---- measurement -------------------------------------------------------- */
+/* Per-kernel timing with HIP events on the context's stream.  While enabled,
+ * every launch is bracketed by events; afqmc_kernel_stats fills up to `cap`
+ * entries (name, launches, total ms) and returns the count in *count. */
+int afqmc_profile(afqmc_ctx* ctx, int enable);
+int afqmc_kernel_stats(afqmc_ctx* ctx, int cap, char (*names)[64], long long* launches, double* total_ms,
+                       int* count);
+
+/* ---- synthetic input (the reference's example afqmc.h5 is not shipped) --- */
+typedef struct {
+  int nmo, naea, nchol;
+  double density;   /* kept fraction of Cholesky entries (1 = dense) */
+  double sigma_L;   /* std of Re and Im of an unthresholded Cholesky entry */
+  double sigma_h;   /* std of the one-body Gaussian before symmetrisation */
+  double dt;        /* time step: Propg1 = expm(-dt/2 h), Spvn *= sqrt(dt) */
+  uint64_t seed;
+  int with_vakbl;   /* also build the compacted two-body matrix (small systems) */
+} afqmc_synth_params;
+
+typedef struct {
+  int nmo, naea, nchol;
+  const double* trial_alpha; /* [M][N] conj(Psi_alpha) */
+  const double* trial_beta;  /* [M][N] conj(Psi_beta) */
+  const double* haj;         /* [2N][M] */
+  const double* propg1;      /* [M][M] */
+  const double* h1;          /* [M][M] one-body Hamiltonian */
+  afqmc_csr_view spvn;       /* [M*M][K], sqrt(dt)-scaled */
+  afqmc_csr_view vakbl;      /* [2NM][2NM] or nnz = 0 */
+} afqmc_synth_view;
+
+typedef struct afqmc_synth afqmc_synth;
+int afqmc_synth_create(const afqmc_synth_params* p, afqmc_synth** out);
+int afqmc_synth_destroy(afqmc_synth* s);
+int afqmc_synth_get(const afqmc_synth* s, afqmc_synth_view* v);
+/* real N(0,1) draws for walkers [w0, w0+nwalk) of a run with wtot walkers,
+ * one substep, written as [K][nwalk] */
+int afqmc_synth_fields(uint64_t seed, int nchol, int nwalk, int w0, int wtot, long substep, double* x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

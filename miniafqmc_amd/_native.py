@@ -1,0 +1,145 @@
+"""ctypes binding of libafqmc_hip.so (the C-ABI declared in include/afqmc_hip.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (``make -C
+miniafqmc_amd/csrc``).  There is no fallback: if the shared library is
+missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafqmc_hip.so")
+
+STATUS = {
+    0: "AFQMC_OK",
+    1: "AFQMC_ERR_ARG",
+    2: "AFQMC_ERR_SHAPE",
+    3: "AFQMC_ERR_ALLOC",
+    4: "AFQMC_ERR_HIP",
+    5: "AFQMC_ERR_SINGULAR",
+    6: "AFQMC_ERR_NONFINITE",
+    7: "AFQMC_ERR_STATE",
+    8: "AFQMC_ERR_UNSUPPORTED",
+}
+
+MODES = {"fp64_csr": 0, "fp64_dense": 1, "mixed_fp32": 2, "mixed_bf16": 3}
+
+BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW = range(7)
+
+# every symbol include/afqmc_hip.h declares
+EXPORTED = [
+    "afqmc_ctx_create", "afqmc_ctx_destroy", "afqmc_ctx_error", "afqmc_sync",
+    "afqmc_upload_system", "afqmc_get_spvnT", "afqmc_init_walkers", "afqmc_copy",
+    "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_fields", "afqmc_vhs",
+    "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
+    "afqmc_weights", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
+    "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run",
+    "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats",
+    "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
+]
+
+
+class CsrView(C.Structure):
+    _fields_ = [
+        ("nrows", C.c_int),
+        ("ncols", C.c_int),
+        ("nnz", C.c_longlong),
+        ("rowptr", C.POINTER(C.c_int)),
+        ("col", C.POINTER(C.c_int)),
+        ("val", C.POINTER(C.c_double)),
+    ]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [
+        ("nmo", C.c_int),
+        ("naea", C.c_int),
+        ("nchol", C.c_int),
+        ("density", C.c_double),
+        ("sigma_L", C.c_double),
+        ("sigma_h", C.c_double),
+        ("dt", C.c_double),
+        ("seed", C.c_uint64),
+        ("with_vakbl", C.c_int),
+    ]
+
+
+class SynthView(C.Structure):
+    _fields_ = [
+        ("nmo", C.c_int),
+        ("naea", C.c_int),
+        ("nchol", C.c_int),
+        ("trial_alpha", C.POINTER(C.c_double)),
+        ("trial_beta", C.POINTER(C.c_double)),
+        ("haj", C.POINTER(C.c_double)),
+        ("propg1", C.POINTER(C.c_double)),
+        ("h1", C.POINTER(C.c_double)),
+        ("spvn", CsrView),
+        ("vakbl", CsrView),
+    ]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C miniafqmc_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    I = C.c_int
+    L = C.c_long
+    sig = {
+        "afqmc_ctx_create": (I, [I, I, I, I, I, C.POINTER(P)]),
+        "afqmc_ctx_destroy": (I, [P]),
+        "afqmc_ctx_error": (C.c_char_p, [P]),
+        "afqmc_sync": (I, [P]),
+        "afqmc_upload_system": (I, [P, D, D, D, D, C.POINTER(CsrView), C.POINTER(CsrView), C.c_double, C.c_double]),
+        "afqmc_get_spvnT": (I, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int), D]),
+        "afqmc_init_walkers": (I, [P]),
+        "afqmc_copy": (I, [P, I, I, D]),
+        "afqmc_mixed_density_matrix": (I, [P]),
+        "afqmc_vbias": (I, [P]),
+        "afqmc_fields": (I, [P, D, L]),
+        "afqmc_vhs": (I, [P]),
+        "afqmc_propagate": (I, [P]),
+        "afqmc_save_overlaps": (I, [P]),
+        "afqmc_calculate_overlaps": (I, [P]),
+        "afqmc_weights": (I, [P, C.c_double, D]),
+        "afqmc_orthogonalize": (I, [P]),
+        "afqmc_energy": (I, [P, D]),
+        "afqmc_stage_fields": (I, [P, D, L]),
+        "afqmc_substep": (I, [P, L, I, C.c_double]),
+        "afqmc_get_eshift": (I, [P, D]),
+        "afqmc_set_eshift": (I, [P, C.c_double]),
+        "afqmc_run": (I, [P, L, I, I, I, C.c_double, D]),
+        "afqmc_use_graphs": (I, [P, I]),
+        "afqmc_profile": (I, [P, I]),
+        "afqmc_kernel_stats": (I, [P, I, P, C.POINTER(C.c_longlong), D, C.POINTER(C.c_int)]),
+        "afqmc_synth_create": (I, [C.POINTER(SynthParams), C.POINTER(P)]),
+        "afqmc_synth_destroy": (I, [P]),
+        "afqmc_synth_get": (I, [P, C.POINTER(SynthView)]),
+        "afqmc_synth_fields": (I, [C.c_uint64, I, I, I, I, L, D]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class AfqmcError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+
+
+def check(code, ctx=None):
+    if code != 0:
+        msg = ""
+        if ctx is not None:
+            m = lib.afqmc_ctx_error(ctx)
+            msg = m.decode() if m else ""
+        raise AfqmcError(code, msg)

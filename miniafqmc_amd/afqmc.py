@@ -1,0 +1,185 @@
+"""Host-side mirror of the reference's walker-step interface over the C-ABI.
+
+``AfqmcSys`` carries the method names of ``base::afqmc_sys``
+(src/AFQMC/afqmc_sys.hpp:44-230) and the driver's free functions
+(``get_vbias``, ``get_vHS``), with the reference's array layouts on the host:
+walkers ``[nwalk][2][M][N]``, ``W_data [nwalk][8]``, compact G
+``[2NM][nwalk]``, ``vbias``/``X`` ``[K][nwalk]``, ``vHS [M*M][nwalk]``.  The
+arithmetic runs on the GPU through libafqmc_hip.so; nothing here computes.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+from .system import System
+
+_D = C.POINTER(C.c_double)
+
+
+def _dp(a):
+    return a.view(np.float64).ctypes.data_as(_D) if a.dtype == np.complex128 else a.ctypes.data_as(_D)
+
+
+class AfqmcSys:
+    def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
+                 use_vakbl: bool = True):
+        self.sys = system
+        self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
+        self.mode = mode
+        h = C.c_void_p()
+        nat.check(nat.lib.afqmc_ctx_create(self.M, self.N, self.K, nwalk, nat.MODES[mode], C.byref(h)))
+        self.ctx = h
+        ta = np.ascontiguousarray(system.trial_alpha)
+        tb = np.ascontiguousarray(system.trial_beta)
+        sp = system.spvn.view()
+        vk = system.vakbl.view() if (use_vakbl and system.vakbl is not None) else None
+        self._check(nat.lib.afqmc_upload_system(
+            self.ctx, _dp(ta), _dp(tb), _dp(np.ascontiguousarray(system.propg1)),
+            _dp(np.ascontiguousarray(system.haj)), C.byref(sp), C.byref(vk) if vk is not None else None,
+            cutoff, system.dt))
+
+    # -- plumbing -------------------------------------------------------------
+    def _check(self, code):
+        nat.check(code, self.ctx)
+
+    def close(self):
+        if self.ctx:
+            nat.lib.afqmc_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self._check(nat.lib.afqmc_sync(self.ctx))
+
+    _shapes = {
+        nat.BUF_W: lambda s: (s.nwalk, 2, s.M, s.N),
+        nat.BUF_WDATA: lambda s: (s.nwalk, 8),
+        nat.BUF_G: lambda s: (2 * s.N * s.M, s.nwalk),
+        nat.BUF_VBIAS: lambda s: (s.K, s.nwalk),
+        nat.BUF_X: lambda s: (s.K, s.nwalk),
+        nat.BUF_VHS: lambda s: (s.M * s.M, s.nwalk),
+        nat.BUF_HYBRIDW: lambda s: (s.nwalk,),
+    }
+
+    def get(self, buf):
+        out = np.empty(self._shapes[buf](self), np.complex128)
+        self._check(nat.lib.afqmc_copy(self.ctx, buf, 0, _dp(out)))
+        return out
+
+    def set(self, buf, arr):
+        a = np.ascontiguousarray(arr, dtype=np.complex128)
+        assert a.shape == self._shapes[buf](self), (a.shape, self._shapes[buf](self))
+        self._check(nat.lib.afqmc_copy(self.ctx, buf, 1, _dp(a)))
+
+    @property
+    def W(self):
+        return self.get(nat.BUF_W)
+
+    @property
+    def W_data(self):
+        return self.get(nat.BUF_WDATA)
+
+    # -- reference interface ---------------------------------------------------
+    def init_walkers(self):
+        """Walkers = conj(trialwfn), weights 1 (miniafqmc_base.cpp:225-236)."""
+        self._check(nat.lib.afqmc_init_walkers(self.ctx))
+
+    def calculate_mixed_density_matrix(self, compact=True):
+        """afqmc_sys::calculate_mixed_density_matrix (afqmc_sys.hpp:95-116); returns G [2NM][nw]."""
+        if not compact:
+            raise nat.AfqmcError(8, "full (non-compact) density matrix")
+        self._check(nat.lib.afqmc_mixed_density_matrix(self.ctx))
+        return self.get(nat.BUF_G)
+
+    def calculate_overlaps(self):
+        """afqmc_sys::calculate_overlaps (afqmc_sys.hpp:135-144)."""
+        self._check(nat.lib.afqmc_calculate_overlaps(self.ctx))
+
+    def get_vbias(self):
+        """base::get_vbias(SpvnT, G, vbias, transposed=true) (vbias.hpp:45-83)."""
+        self._check(nat.lib.afqmc_vbias(self.ctx))
+        return self.get(nat.BUF_VBIAS)
+
+    def fields(self, x):
+        """X = x + i vbias and the hybrid weight (miniafqmc_base.cpp:281-289)."""
+        xa = np.ascontiguousarray(x, dtype=np.float64)
+        assert xa.shape == (self.K, self.nwalk)
+        self._check(nat.lib.afqmc_fields(self.ctx, _dp(xa), 0))
+
+    def get_vHS(self):
+        """base::get_vHS(Spvn, X, vHS) (vHS.hpp:43-56)."""
+        self._check(nat.lib.afqmc_vhs(self.ctx))
+        return self.get(nat.BUF_VHS)
+
+    def propagate(self):
+        """afqmc_sys::propagate (afqmc_sys.hpp:146-173)."""
+        self._check(nat.lib.afqmc_propagate(self.ctx))
+
+    def save_overlaps(self):
+        self._check(nat.lib.afqmc_save_overlaps(self.ctx))
+
+    def weights(self, dt, eshift):
+        e = C.c_double(eshift)
+        self._check(nat.lib.afqmc_weights(self.ctx, dt, C.byref(e)))
+        return e.value
+
+    def orthogonalize(self):
+        """afqmc_sys::orthogonalize (afqmc_sys.hpp:175-207)."""
+        self._check(nat.lib.afqmc_orthogonalize(self.ctx))
+
+    def calculate_energy(self):
+        """afqmc_sys::calculate_energy (afqmc_sys.hpp:118-133) on the current G."""
+        e = C.c_double(0.0)
+        self._check(nat.lib.afqmc_energy(self.ctx, C.byref(e)))
+        return e.value
+
+    def spvnT(self):
+        """Half-rotated Cholesky matrix built at upload (rotate.hpp:65-169)."""
+        from .system import Csr
+        nnz = C.c_longlong(0)
+        self._check(nat.lib.afqmc_get_spvnT(self.ctx, C.byref(nnz), None, None, None))
+        rp = np.empty(self.K + 1, np.int32)
+        col = np.empty(nnz.value, np.int32)
+        val = np.empty(nnz.value, np.complex128)
+        self._check(nat.lib.afqmc_get_spvnT(self.ctx, C.byref(nnz), rp.ctypes.data_as(C.POINTER(C.c_int)),
+                                            col.ctypes.data_as(C.POINTER(C.c_int)), _dp(val)))
+        return Csr(self.K, 2 * self.N * self.M, rp, col, val)
+
+    # -- whole loop --------------------------------------------------------------
+    def stage_fields(self, x_all):
+        xa = np.ascontiguousarray(x_all, dtype=np.float64)
+        assert xa.ndim == 3 and xa.shape[1:] == (self.K, self.nwalk)
+        self._check(nat.lib.afqmc_stage_fields(self.ctx, _dp(xa), xa.shape[0]))
+
+    def substep(self, s, do_ortho, dt):
+        self._check(nat.lib.afqmc_substep(self.ctx, s, int(do_ortho), dt))
+
+    def eshift(self):
+        e = C.c_double(0)
+        self._check(nat.lib.afqmc_get_eshift(self.ctx, C.byref(e)))
+        return e.value
+
+    def run(self, nsteps, nsubsteps, northo, dt, first=0, energy=True):
+        """The driver's step loop (miniafqmc_base.cpp:248-350); returns Eav per step."""
+        eav = np.zeros(max(nsteps, 1), np.float64)
+        self._check(nat.lib.afqmc_run(self.ctx, first, nsteps, nsubsteps, northo, dt,
+                                      _dp(eav) if energy else None))
+        return eav[:nsteps]
+
+    def profile(self, on=True):
+        self._check(nat.lib.afqmc_profile(self.ctx, int(on)))
+
+    def kernel_stats(self):
+        cap = 128
+        names = (C.c_char * 64 * cap)()
+        n = (C.c_longlong * cap)()
+        ms = (C.c_double * cap)()
+        cnt = C.c_int(0)
+        self._check(nat.lib.afqmc_kernel_stats(self.ctx, cap, C.cast(names, C.c_void_p), n, ms, C.byref(cnt)))
+        return {names[i].value.decode(): (int(n[i]), float(ms[i])) for i in range(min(cnt.value, cap))}

@@ -1,0 +1,56 @@
+// Shared device/host definitions for the gfx950 walker-step kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef short shortx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct zd {
+  double re, im;
+};
+struct zf {
+  float re, im;
+};
+
+__host__ __device__ inline zd zmul(zd a, zd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__host__ __device__ inline zd zadd(zd a, zd b) { return {a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline zd zsub(zd a, zd b) { return {a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline zd zconj(zd a) { return {a.re, -a.im}; }
+__host__ __device__ inline double zcabs1(zd a) { return fabs(a.re) + fabs(a.im); }
+// Smith's complex division
+__host__ __device__ inline zd zdiv(zd a, zd b)
+{
+  if (fabs(b.im) <= fabs(b.re)) {
+    double r = b.im / b.re, den = b.re + b.im * r;
+    return {(a.re + a.im * r) / den, (a.im - a.re * r) / den};
+  }
+  double r = b.re / b.im, den = b.im + b.re * r;
+  return {(a.re * r + a.im) / den, (a.im * r - a.re) / den};
+}
+
+// round-to-nearest-even float -> bf16 bit pattern (inputs are finite)
+__device__ inline unsigned short f2bf(float x)
+{
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short b) { return __uint_as_float(((uint32_t)b) << 16); }
+
+template <class T> struct ztype;
+template <> struct ztype<double> {
+  using type = zd;
+};
+template <> struct ztype<float> {
+  using type = zf;
+};
+
+__device__ inline zd load_z(const zd* p) { return *p; }
+__device__ inline zd load_z(const zf* p)
+{
+  zf v = *p;
+  return {(double)v.re, (double)v.im};
+}

@@ -1,0 +1,1199 @@
+// Context, device memory plan and phase orchestration behind afqmc_hip.h.
+//
+// Device layouts (HBM):
+//   AFQMC_FP64_CSR: the reference's own layouts (walker index last for
+//     G [2NM][W], vbias/X [K][W], vHS [M*M][W]) so every csrmm reads rows of
+//     walkers contiguously; vHS is transposed once per substep to [W][M*M]
+//     for the per-walker propagation.
+//   dense modes: walker index first everywhere (G [W][2][N][M],
+//     vbias/X [W][K], vHS [W][M][M]) so each contraction is a GEMM whose
+//     operands are contiguous along the summed index; the Cholesky vectors
+//     are kept as Ld [K][M*M] (vector-major) and the half-rotated
+//     Lt [2][K][N*M] (spin-major).
+#include "afqmc_hip.h"
+#include "common.hpp"
+#include "gemm.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct KStat {
+  long long n = 0;
+  double ms = 0.0;
+};
+
+struct EvPair {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+} // namespace
+
+struct afqmc_ctx {
+  int M = 0, N = 0, K = 0, nw = 0, mode = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool have_system = false;
+  bool mixed = false;   // FP32 or split-BF16 contractions
+  double dt_sys = 0.0;  // dt the system was scaled with
+  std::vector<void*> allocs;
+
+  // system
+  zd* trial = nullptr;   // [2][M][N] conj(Psi)
+  zd* propg = nullptr;   // [M][M]
+  zf* propg32 = nullptr; // [M][M] (mixed)
+  zd* haj = nullptr;     // [2N][M]
+  int* sp_rp = nullptr;
+  int* sp_col = nullptr;
+  zd* sp_val = nullptr;
+  long long sp_nnz = 0;
+  int* spt_rp = nullptr;
+  int* spt_col = nullptr;
+  zd* spt_val = nullptr;
+  long long spt_nnz = 0;
+  int* v_rp = nullptr;
+  int* v_col = nullptr;
+  zd* v_val = nullptr;
+  long long v_nnz = 0;
+  void* Ld = nullptr;  // dense modes: [K][M*M]  (zd or zf)
+  void* Lt = nullptr;  // [2][K][N*M] (zd or zf); scaled by sqrt(dt)
+
+  // walker state and per-substep buffers
+  zd* W = nullptr;
+  zd* Wd = nullptr;
+  zd* T1 = nullptr;       // [nw][2][N][N]
+  void* G = nullptr;      // CSR: zd [2NM][nw]; FP64_DENSE: zd [nw][2NM]; mixed: zf [nw][2NM]
+  zd* vbias = nullptr;    // CSR: [K][nw]; dense: [nw][K]
+  void* X = nullptr;      // CSR: zd [K][nw]; dense: [nw][K] (zd / zf)
+  void* vHS = nullptr;    // CSR: zd [M*M][nw]; dense: [nw][M*M] (zd / zf)
+  zd* vHSw = nullptr;     // CSR: [nw][M*M]
+  zd* hybridW = nullptr;
+  void* S = nullptr;      // propagation scratch [nw][2][M][N] x 4 (zd / zf)
+  void* Ta = nullptr;
+  void* Tb = nullptr;
+  void* Sacc = nullptr;
+  double* eshift = nullptr;
+  int* flags = nullptr;   // [0] singular count, [1] non-finite fields
+  double* xstage = nullptr;
+  long nstage = 0;
+  // energy scratch
+  zd* e_direct = nullptr;
+  zd* e_exch = nullptr;
+  zd* e_one = nullptr;
+  zd* e_v = nullptr;      // [nw][K] direct-term contractions
+  void* Ybuf = nullptr;
+  long ybuf_elems = 0;
+  zd* Gtmp = nullptr;     // CSR mode Cholesky energy: G transposed to [nw][2NM]
+  zd* Gcloc = nullptr;    // CSR mode Vakbl energy
+
+  // profiling
+  bool prof = false;
+  std::vector<EvPair> pending;
+  std::vector<hipEvent_t> evpool;
+  std::map<std::string, KStat> stats;
+
+  // graphs
+  bool use_graphs = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  long g_substep = -1;
+  double g_dt = 0.0;
+  double* d_dt = nullptr;
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                        \
+      return AFQMC_ERR_HIP;                                                                   \
+    }                                                                                         \
+  } while (0)
+
+#define RET(call)                   \
+  do {                              \
+    int r_ = (call);                \
+    if (r_ != AFQMC_OK) return r_;  \
+  } while (0)
+
+int fail(afqmc_ctx* c, int code, const std::string& m)
+{
+  c->err = m;
+  return code;
+}
+
+template <class T> int dalloc(afqmc_ctx* c, T** p, size_t n)
+{
+  void* q = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return AFQMC_OK;
+}
+int dalloc_bytes(afqmc_ctx* c, void** p, size_t bytes)
+{
+  char* q = nullptr;
+  RET(dalloc(c, &q, bytes));
+  *p = q;
+  return AFQMC_OK;
+}
+void dfree(afqmc_ctx* c, void* p)
+{
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+hipEvent_t get_event(afqmc_ctx* c)
+{
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Launch wrapper: brackets the launch with events when profiling.
+template <class F> int launch(afqmc_ctx* c, const char* name, F&& f)
+{
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->prof) {
+    a = get_event(c);
+    b = get_event(c);
+    (void)hipEventRecord(a, c->stream);
+  }
+  f();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, std::string(name) + ": " + hipGetErrorString(e));
+  if (c->prof) {
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({name, a, b});
+  }
+  return AFQMC_OK;
+}
+
+void collect_events(afqmc_ctx* c)
+{
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    auto& s = c->stats[p.name];
+    s.n++;
+    s.ms += ms;
+    c->evpool.push_back(p.a);
+    c->evpool.push_back(p.b);
+  }
+  c->pending.clear();
+}
+
+inline int nblk(long n, int t) { return (int)std::min<long>((n + t - 1) / t, 65535L * 8); }
+
+// ---------------------------------------------------------------------------
+// GEMM dispatch.  Small problems use a 1-wave 32x16 tile, large ones a 4-wave
+// 128x64 tile (complex columns).
+template <int CP, class AT, class BT, class OT, bool AK, bool BK>
+int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
+{
+  if (a.m <= 0 || a.n <= 0 || batch <= 0) return AFQMC_OK;
+  if (a.bdiv <= 0) a.bdiv = 1;
+  const bool large = (a.m >= 96 && a.n >= 48);
+  constexpr int SK = (CP == CP_BF16X3) ? 16 : 8;
+  if (large) {
+    constexpr int BM = 128, BNC = 64, WM = 2, WN = 2;
+    dim3 grid((a.n + BNC - 1) / BNC, (a.m + BM - 1) / BM, batch);
+    return launch(c, name, [&]() {
+      hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, BM, BNC, SK, WM, WN>), grid, dim3(64 * WM * WN), 0,
+                         c->stream, a);
+    });
+  } else {
+    constexpr int BM = 32, BNC = 16, WM = 1, WN = 1;
+    constexpr int BKC = (CP == CP_BF16X3) ? 16 : 16;
+    dim3 grid((a.n + BNC - 1) / BNC, (a.m + BM - 1) / BM, batch);
+    return launch(c, name, [&]() {
+      hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, BM, BNC, BKC, WM, WN>), grid, dim3(64 * WM * WN), 0,
+                         c->stream, a);
+    });
+  }
+}
+
+ZGemmArgs gargs()
+{
+  ZGemmArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.bdiv = 1;
+  a.ar = 1.0;
+  return a;
+}
+
+// Batched inverse + determinant of the nw*2 overlap matrices in c->T1.
+int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
+{
+  const int N = c->N;
+  const long batch = 2L * c->nw;
+  const long ms = (long)N * N;
+  if (N <= 16)
+    return launch(c, "gj_inverse", [&]() {
+      hipLaunchKernelGGL((gj_inverse_kernel<16, 4>), dim3(batch), dim3(64), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+                         c->flags);
+    });
+  if (N <= 32)
+    return launch(c, "gj_inverse", [&]() {
+      hipLaunchKernelGGL((gj_inverse_kernel<32, 8>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+                         c->flags);
+    });
+  if (N <= 64)
+    return launch(c, "gj_inverse", [&]() {
+      hipLaunchKernelGGL((gj_inverse_kernel<64, 8>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+                         c->flags);
+    });
+  if (N <= 128)
+    return launch(c, "gj_inverse", [&]() {
+      hipLaunchKernelGGL((gj_inverse_kernel<128, 8>), dim3(batch), dim3(1024), 0, c->stream, T, Tinv, det, dq, dr, N,
+                         ms, c->flags);
+    });
+  return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 128 electrons per spin is not supported by the batched inverse");
+}
+
+// T1[w][s] = W[w][s]^T * trial_s  (N x N, k = M), all walkers and spins
+int overlap_matrices(afqmc_ctx* c)
+{
+  const int M = c->M, N = c->N;
+  ZGemmArgs a = gargs();
+  a.A = c->W;
+  a.sam = 1;
+  a.sak = N;
+  a.sa1 = 2L * M * N;
+  a.sa2 = (long)M * N;
+  a.B = c->trial;
+  a.sbn = 1;
+  a.sbk = N;
+  a.sb1 = 0;
+  a.sb2 = (long)M * N;
+  a.C = c->T1;
+  a.scm = N;
+  a.scn = 1;
+  a.sc1 = 2L * N * N;
+  a.sc2 = (long)N * N;
+  a.bdiv = 2;
+  a.m = N;
+  a.n = N;
+  a.k = M;
+  return zgemm<CP_F64, double, double, double, false, false>(c, "ovlp_gemm", a, 2 * c->nw);
+}
+
+// ---------------------------------------------------------------------------
+int do_mdm(afqmc_ctx* c)
+{
+  const int M = c->M, N = c->N, nw = c->nw;
+  RET(overlap_matrices(c));
+  RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
+  // Gc(a,k) = sum_b T1inv(a,b) W(k,b)
+  ZGemmArgs a = gargs();
+  a.A = c->T1;
+  a.sam = N;
+  a.sak = 1;
+  a.sa1 = 2L * N * N;
+  a.sa2 = (long)N * N;
+  a.B = c->W;
+  a.sbn = N;
+  a.sbk = 1;
+  a.sb1 = 2L * M * N;
+  a.sb2 = (long)M * N;
+  a.bdiv = 2;
+  a.m = N;
+  a.n = M;
+  a.k = N;
+  a.C = c->G;
+  if (c->mode == AFQMC_FP64_CSR) {
+    // G[(s*N + a)*M + k][w]
+    a.scm = (long)M * nw;
+    a.scn = nw;
+    a.sc1 = 1;
+    a.sc2 = (long)N * M * nw;
+    return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
+  }
+  a.scm = M;
+  a.scn = 1;
+  a.sc1 = 2L * N * M;
+  a.sc2 = (long)N * M;
+  if (c->mixed) return zgemm<CP_F64, double, double, float, true, true>(c, "gc_gemm", a, 2 * nw);
+  return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
+}
+
+int do_overlaps(afqmc_ctx* c)
+{
+  RET(overlap_matrices(c));
+  return gj_inverse(c, c->T1, nullptr, c->Wd + 2, 8, 1);
+}
+
+// vbias(w, n) = sum_{s,j} G(w, s, j) Lt(s, n, j) in dense modes.
+template <int CP, class GT, class LT_>
+int dense_contract_vbias(afqmc_ctx* c, const void* Gp, zd* out, const char* name)
+{
+  const long NM = (long)c->N * c->M;
+  for (int sp = 0; sp < 2; sp++) {
+    ZGemmArgs a = gargs();
+    a.A = (const GT*)Gp + 2 * sp * NM;
+    a.sam = 2 * NM;
+    a.sak = 1;
+    a.B = (const LT_*)c->Lt + 2 * (sp * (long)c->K * NM);
+    a.sbn = NM;
+    a.sbk = 1;
+    a.C = out;
+    a.scm = c->K;
+    a.scn = 1;
+    a.m = c->nw;
+    a.n = c->K;
+    a.k = (int)NM;
+    a.br = sp == 0 ? 0.0 : 1.0;
+    RET((zgemm<CP, GT, LT_, double, true, true>(c, name, a, 1)));
+  }
+  return AFQMC_OK;
+}
+
+int do_vbias(afqmc_ctx* c)
+{
+  const int nw = c->nw, K = c->K, NAK = 2 * c->N * c->M;
+  if (c->mode == AFQMC_FP64_CSR) {
+    zd one{1, 0}, zero{0, 0};
+    long waves = (long)K * ((nw + 63) / 64);
+    return launch(c, "csrmm_vbias", [&]() {
+      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, K, nw, NAK, c->spt_rp,
+                         c->spt_col, c->spt_val, (const zd*)c->G, (long)nw, c->vbias, (long)nw, one, zero);
+    });
+  }
+  if (c->mode == AFQMC_FP64_DENSE) return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->vbias, "vbias_gemm");
+  if (c->mode == AFQMC_MIXED_FP32) return dense_contract_vbias<CP_F32, float, float>(c, c->G, c->vbias, "vbias_gemm");
+  return dense_contract_vbias<CP_BF16X3, float, float>(c, c->G, c->vbias, "vbias_gemm");
+}
+
+int do_fields(afqmc_ctx* c, const double* xdev)
+{
+  const int nw = c->nw, K = c->K;
+  dim3 grid((nw + 63) / 64);
+  if (c->mode == AFQMC_FP64_CSR)
+    return launch(c, "fields", [&]() {
+      hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, (long)nw, 1L, xdev,
+                         (zd*)c->X, (long)nw, 1L, c->hybridW, c->flags + 1);
+    });
+  if (!c->mixed)
+    return launch(c, "fields", [&]() {
+      hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
+                         (zd*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+    });
+  return launch(c, "fields", [&]() {
+    hipLaunchKernelGGL((fields_kernel<zd, zf>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
+                       (zf*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+  });
+}
+
+int do_vhs(afqmc_ctx* c)
+{
+  const int nw = c->nw, K = c->K;
+  const long MM = (long)c->M * c->M;
+  if (c->mode == AFQMC_FP64_CSR) {
+    zd one{1, 0}, zero{0, 0};
+    long waves = MM * ((nw + 63) / 64);
+    RET(launch(c, "csrmm_vhs", [&]() {
+      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, (int)MM, nw, K, c->sp_rp,
+                         c->sp_col, c->sp_val, (const zd*)c->X, (long)nw, (zd*)c->vHS, (long)nw, one, zero);
+    }));
+    dim3 grid((nw + 31) / 32, (unsigned)((MM + 31) / 32));
+    return launch(c, "vhs_transpose", [&]() {
+      hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->vHS, c->vHSw,
+                         (int)MM, nw, (long)nw, MM);
+    });
+  }
+  // vHS(w, ik) = sum_n X(w, n) Ld(n, ik)
+  ZGemmArgs a = gargs();
+  a.A = c->X;
+  a.sam = K;
+  a.sak = 1;
+  a.B = c->Ld;
+  a.sbn = 1;
+  a.sbk = MM;
+  a.C = c->vHS;
+  a.scm = MM;
+  a.scn = 1;
+  a.m = nw;
+  a.n = (int)MM;
+  a.k = K;
+  if (c->mode == AFQMC_FP64_DENSE) return zgemm<CP_F64, double, double, double, true, false>(c, "vhs_gemm", a, 1);
+  if (c->mode == AFQMC_MIXED_FP32) return zgemm<CP_F32, float, float, float, true, false>(c, "vhs_gemm", a, 1);
+  return zgemm<CP_BF16X3, float, float, float, true, false>(c, "vhs_gemm", a, 1);
+}
+
+// W <- P * T6(V) * P * W per walker and spin (afqmc_sys.hpp:146-173), T6 the
+// order-6 Taylor polynomial of exp(i V) applied as in apply_expM
+// (vHS.hpp:61-91): T_n = (i/n) V T_{n-1}, S += T_n.
+template <int CP, class ST, class PT, class VT>
+int propagate_chain(afqmc_ctx* c, const PT* P, const VT* V)
+{
+  const int M = c->M, N = c->N, nw = c->nw;
+  const long MN = (long)M * N, MM = (long)M * M;
+  auto base = [&]() {
+    ZGemmArgs a = gargs();
+    a.sam = M;
+    a.sak = 1;
+    a.sbn = 1;
+    a.sbk = N;
+    a.sb1 = 2 * MN;
+    a.sb2 = MN;
+    a.scm = N;
+    a.scn = 1;
+    a.sc1 = 2 * MN;
+    a.sc2 = MN;
+    a.bdiv = 2;
+    a.m = M;
+    a.n = N;
+    a.k = M;
+    return a;
+  };
+  // S = P * W
+  {
+    ZGemmArgs a = base();
+    a.A = P;
+    a.B = c->W;
+    a.C = c->S;
+    RET((zgemm<CP, PT, double, ST, true, false>(c, "prop_pw", a, 2 * nw)));
+  }
+  const void* prev = c->S;
+  void* bufs[2] = {c->Ta, c->Tb};
+  for (int n = 1; n <= 6; n++) {
+    ZGemmArgs a = base();
+    a.A = V;
+    a.sa1 = MM;
+    a.sa2 = 0;
+    a.B = prev;
+    a.C = (n < 6) ? bufs[n & 1] : nullptr;
+    a.ar = 0.0;
+    a.ai = 1.0 / (double)n;
+    a.D = c->Sacc;
+    a.F = (n == 1) ? c->S : c->Sacc;
+    RET((zgemm<CP, VT, ST, ST, true, false>(c, "prop_taylor", a, 2 * nw)));
+    prev = bufs[n & 1];
+  }
+  // W = P * S
+  {
+    ZGemmArgs a = base();
+    a.A = P;
+    a.B = c->Sacc;
+    a.C = c->W;
+    RET((zgemm<CP, PT, ST, double, true, false>(c, "prop_ps", a, 2 * nw)));
+  }
+  return AFQMC_OK;
+}
+
+int do_propagate(afqmc_ctx* c)
+{
+  if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHSw);
+  if (c->mode == AFQMC_FP64_DENSE)
+    return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHS);
+  if (c->mode == AFQMC_MIXED_FP32)
+    return propagate_chain<CP_F32, float, float, float>(c, (const float*)c->propg32, (const float*)c->vHS);
+  return propagate_chain<CP_BF16X3, float, float, float>(c, (const float*)c->propg32, (const float*)c->vHS);
+}
+
+int do_save_overlaps(afqmc_ctx* c)
+{
+  return launch(c, "save_overlaps", [&]() {
+    hipLaunchKernelGGL(save_overlaps_kernel, dim3((c->nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->nw);
+  });
+}
+
+int do_weights(afqmc_ctx* c, const double* dtp, double dt)
+{
+  (void)dtp;
+  int th = 64;
+  while (th < c->nw && th < 1024) th *= 2;  // power of two for the tree reduction
+  return launch(c, "weights", [&]() {
+    hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(th), 0, c->stream, c->Wd, c->hybridW, c->nw, dt, c->eshift);
+  });
+}
+
+int do_ortho(afqmc_ctx* c)
+{
+  return launch(c, "lq_orthogonalize", [&]() {
+    hipLaunchKernelGGL(lq_kernel, dim3(2 * c->nw), dim3(256), c->N * sizeof(zd), c->stream, c->W, c->M, c->N);
+  });
+}
+
+// Cholesky-form two-body energy from G in [nw][2][N][M] layout (type GT) and
+// Lt [2][K][NM] (type LT_, scaled by sqrt(dt)).
+template <int CP, class GT, class LT_, class YT>
+int energy_cholesky(afqmc_ctx* c, const void* Gp)
+{
+  const int M = c->M, N = c->N, K = c->K, nw = c->nw;
+  const long NM = (long)N * M;
+  const double inv_dt = 1.0 / c->dt_sys;
+  RET(launch(c, "energy_zero", [&]() {
+    (void)hipMemsetAsync(c->e_direct, 0, sizeof(zd) * nw, c->stream);
+    (void)hipMemsetAsync(c->e_exch, 0, sizeof(zd) * nw, c->stream);
+  }));
+  // direct term: v(w,n) = sum Lt G ; direct = sum_n v^2 / dt
+  RET((dense_contract_vbias<CP, GT, LT_>(c, Gp, c->e_v, "energy_v_gemm")));
+  RET(launch(c, "energy_direct", [&]() {
+    hipLaunchKernelGGL((sumsq_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, c->e_v, 1L, (long)K, c->e_direct,
+                       inv_dt);
+  }));
+  // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba; vectors in chunks
+  const long per_n = (long)N * N;
+  int nq = (int)std::min<long>(K, std::max<long>(1, c->ybuf_elems / per_n));
+  for (int w = 0; w < nw; w++) {
+    for (int sp = 0; sp < 2; sp++) {
+      for (int n0 = 0; n0 < K; n0 += nq) {
+        int cnt = std::min(nq, K - n0);
+        ZGemmArgs a = gargs();
+        a.A = (const LT_*)c->Lt + 2 * ((long)sp * K * NM + (long)n0 * NM);
+        a.sam = M;
+        a.sak = 1;
+        a.B = (const GT*)Gp + 2 * ((long)w * 2 * NM + sp * NM);
+        a.sbn = M;
+        a.sbk = 1;
+        a.C = c->Ybuf;
+        a.scm = N;
+        a.scn = 1;
+        a.m = cnt * N;
+        a.n = N;
+        a.k = M;
+        a.ar = std::sqrt(inv_dt);  // unscale both factors: Y = Y_scaled / sqrt(dt)... applied once to Y
+        RET((zgemm<CP, LT_, GT, YT, true, true>(c, "energy_y_gemm", a, 1)));
+        RET(launch(c, "energy_trace", [&]() {
+          hipLaunchKernelGGL((trace_sq_kernel<typename ztype<YT>::type>), dim3(1), dim3(256), 0, c->stream,
+                             (const typename ztype<YT>::type*)c->Ybuf, N, cnt, 0L, c->e_exch + w, 0L, 1);
+        }));
+      }
+    }
+  }
+  // one-body: e1 = sum_i G(w,i) haj(i)
+  RET(launch(c, "energy_one", [&]() {
+    hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, zd>), dim3(nw), dim3(256), 0, c->stream, (int)(2 * NM),
+                       (const typename ztype<GT>::type*)Gp, 1L, 2 * NM, (const zd*)c->haj, 1L, 0L, c->e_one, 1L, 1.0,
+                       0);
+  }));
+  return launch(c, "energy_assemble", [&]() {
+    hipLaunchKernelGGL(energy_assemble_kernel, dim3((nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->e_direct,
+                       c->e_exch, c->e_one, nw);
+  });
+}
+
+int do_energy(afqmc_ctx* c, double* eav_out)
+{
+  const int nw = c->nw, N = c->N, M = c->M;
+  const long NAK = 2L * N * M;
+  if (c->mode == AFQMC_FP64_CSR) {
+    if (c->v_rp) {
+      zd one{1, 0}, zero{0, 0};
+      long waves = NAK * ((nw + 63) / 64);
+      RET(launch(c, "csrmm_vakbl", [&]() {
+        hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, (int)NAK, nw, (int)NAK,
+                           c->v_rp, c->v_col, c->v_val, (const zd*)c->G, (long)nw, c->Gcloc, (long)nw, one, zero);
+      }));
+      RET(launch(c, "energy_two", [&]() {
+        hipLaunchKernelGGL((coldot_kernel<zd, zd>), dim3(nw), dim3(256), 0, c->stream, (int)NAK, (const zd*)c->G,
+                           (long)nw, 1L, (const zd*)c->Gcloc, (long)nw, 1L, c->Wd, 8L, 0.5, 0);
+      }));
+      RET(launch(c, "energy_one", [&]() {
+        hipLaunchKernelGGL((coldot_kernel<zd, zd>), dim3(nw), dim3(256), 0, c->stream, (int)NAK, (const zd*)c->G,
+                           (long)nw, 1L, (const zd*)c->haj, 1L, 0L, c->Wd, 8L, 1.0, 1);
+      }));
+    } else {
+      dim3 grid((nw + 31) / 32, (unsigned)((NAK + 31) / 32));
+      RET(launch(c, "g_transpose", [&]() {
+        hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->G, c->Gtmp,
+                           (int)NAK, nw, (long)nw, NAK);
+      }));
+      RET((energy_cholesky<CP_F64, double, double, double>(c, c->Gtmp)));
+    }
+  } else if (c->mode == AFQMC_FP64_DENSE) {
+    RET((energy_cholesky<CP_F64, double, double, double>(c, c->G)));
+  } else if (c->mode == AFQMC_MIXED_FP32) {
+    RET((energy_cholesky<CP_F32, float, float, float>(c, c->G)));
+  } else {
+    RET((energy_cholesky<CP_BF16X3, float, float, float>(c, c->G)));
+  }
+  if (eav_out) {
+    std::vector<zd> wd((size_t)nw * 8);
+    HIPCHK(c, hipMemcpyAsync(wd.data(), c->Wd, sizeof(zd) * wd.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double eav = 0, wgt = 0;
+    for (int n = 0; n < nw; n++) {
+      wgt += wd[(size_t)n * 8 + 1].re;
+      eav += wd[(size_t)n * 8].re * wd[(size_t)n * 8 + 1].re;
+    }
+    *eav_out = eav / wgt;
+  }
+  return AFQMC_OK;
+}
+
+int check_flags(afqmc_ctx* c)
+{
+  int f[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(f, c->flags, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (f[1]) {
+    HIPCHK(c, hipMemsetAsync(c->flags + 1, 0, sizeof(int), c->stream));
+    return fail(c, AFQMC_ERR_NONFINITE, "non-finite auxiliary field draw");
+  }
+  if (f[0]) {
+    HIPCHK(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+    return fail(c, AFQMC_ERR_SINGULAR, "singular overlap matrix (zero pivot)");
+  }
+  return AFQMC_OK;
+}
+
+int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt)
+{
+  RET(do_mdm(c));
+  RET(do_vbias(c));
+  RET(do_fields(c, xdev));
+  RET(do_vhs(c));
+  RET(do_propagate(c));
+  RET(do_save_overlaps(c));
+  RET(do_overlaps(c));
+  RET(do_weights(c, nullptr, dt));
+  if (do_ortho_) {
+    RET(do_ortho(c));
+    RET(do_overlaps(c));
+  }
+  return AFQMC_OK;
+}
+
+bool dense_layout(const afqmc_ctx* c) { return c->mode != AFQMC_FP64_CSR; }
+
+// host transpose helpers: ref [R][C] <-> dev [C][R]
+void htranspose(const zd* in, zd* out, long R, long Cc)
+{
+  for (long r = 0; r < R; r++)
+    for (long cc = 0; cc < Cc; cc++) out[cc * R + r] = in[r * Cc + cc];
+}
+
+} // namespace
+
+extern "C" {
+
+int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ctx** out)
+{
+  if (!out) return AFQMC_ERR_ARG;
+  *out = nullptr;
+  if (nmo <= 0 || naea <= 0 || naea > nmo || nchol <= 0 || nwalk <= 0) return AFQMC_ERR_SHAPE;
+  if (mode < AFQMC_FP64_CSR || mode > AFQMC_MIXED_BF16) return AFQMC_ERR_ARG;
+  if (naea > 128) return AFQMC_ERR_UNSUPPORTED;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AFQMC_ERR_HIP;
+  afqmc_ctx* c = new afqmc_ctx;
+  c->M = nmo;
+  c->N = naea;
+  c->K = nchol;
+  c->nw = nwalk;
+  c->mode = mode;
+  c->mixed = (mode == AFQMC_MIXED_FP32 || mode == AFQMC_MIXED_BF16);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return AFQMC_ERR_HIP;
+  }
+  const long M = nmo, N = naea, K = nchol, nw = nwalk;
+  const long MN = M * N, MM = M * M;
+  const size_t zs = c->mixed ? sizeof(zf) : sizeof(zd);
+  int r = AFQMC_OK;
+  auto chk = [&](int x) {
+    if (r == AFQMC_OK) r = x;
+  };
+  chk(dalloc(c, &c->W, nw * 2 * MN));
+  chk(dalloc(c, &c->Wd, nw * 8));
+  chk(dalloc(c, &c->T1, nw * 2 * N * N));
+  chk(dalloc_bytes(c, &c->G, nw * 2 * MN * zs));
+  chk(dalloc(c, &c->vbias, nw * K));
+  chk(dalloc_bytes(c, &c->X, nw * K * zs));
+  chk(dalloc_bytes(c, &c->vHS, nw * MM * zs));
+  if (mode == AFQMC_FP64_CSR) chk(dalloc(c, &c->vHSw, nw * MM));
+  chk(dalloc(c, &c->hybridW, nw));
+  chk(dalloc_bytes(c, &c->S, nw * 2 * MN * zs));
+  chk(dalloc_bytes(c, &c->Ta, nw * 2 * MN * zs));
+  chk(dalloc_bytes(c, &c->Tb, nw * 2 * MN * zs));
+  chk(dalloc_bytes(c, &c->Sacc, nw * 2 * MN * zs));
+  chk(dalloc(c, &c->eshift, 1));
+  chk(dalloc(c, &c->flags, 4));
+  chk(dalloc(c, &c->e_direct, nw));
+  chk(dalloc(c, &c->e_exch, nw));
+  chk(dalloc(c, &c->e_one, nw));
+  chk(dalloc(c, &c->e_v, nw * K));
+  c->ybuf_elems = std::max<long>(N * N, std::min<long>(K * N * N, 1L << 24));
+  chk(dalloc_bytes(c, &c->Ybuf, c->ybuf_elems * zs));
+  if (r != AFQMC_OK) {
+    afqmc_ctx_destroy(c);
+    return r;
+  }
+  (void)hipMemsetAsync(c->eshift, 0, sizeof(double), c->stream);
+  (void)hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream);
+  (void)hipMemsetAsync(c->Wd, 0, sizeof(zd) * nw * 8, c->stream);
+  *out = c;
+  return AFQMC_OK;
+}
+
+int afqmc_ctx_destroy(afqmc_ctx* c)
+{
+  if (!c) return AFQMC_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int i = 0; i < 2; i++)
+    if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+  for (auto& p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->evpool) (void)hipEventDestroy(e);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AFQMC_OK;
+}
+
+const char* afqmc_ctx_error(const afqmc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int afqmc_sync(afqmc_ctx* c)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* trial_beta, const double* propg1,
+                        const double* haj, const afqmc_csr_view* spvn, const afqmc_csr_view* vakbl, double cutoff,
+                        double dt)
+{
+  if (!c || !trial_alpha || !trial_beta || !propg1 || !haj || !spvn) return AFQMC_ERR_ARG;
+  const long M = c->M, N = c->N, K = c->K;
+  const long MN = M * N, MM = M * M, NM = MN;
+  if (spvn->nrows != MM || spvn->ncols != K || spvn->nnz < 0 || !spvn->rowptr || (spvn->nnz > 0 && (!spvn->col || !spvn->val)))
+    return fail(c, AFQMC_ERR_SHAPE, "Spvn must be CSR [M*M][K]");
+  if (spvn->rowptr[MM] - spvn->rowptr[0] != spvn->nnz) return fail(c, AFQMC_ERR_SHAPE, "Spvn rowptr/nnz mismatch");
+  if (vakbl && vakbl->nnz > 0 && (vakbl->nrows != 2 * NM || vakbl->ncols != 2 * NM))
+    return fail(c, AFQMC_ERR_SHAPE, "Vakbl must be CSR [2NM][2NM]");
+  if (dt <= 0.0) return fail(c, AFQMC_ERR_ARG, "dt must be positive");
+  if (c->have_system) return fail(c, AFQMC_ERR_STATE, "system already uploaded");
+  c->dt_sys = dt;
+  hipStream_t st = c->stream;
+  RET(dalloc(c, &c->trial, 2 * MN));
+  RET(dalloc(c, &c->propg, MM));
+  RET(dalloc(c, &c->haj, 2 * NM));
+  HIPCHK(c, hipMemcpyAsync(c->trial, trial_alpha, sizeof(zd) * MN, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->trial + MN, trial_beta, sizeof(zd) * MN, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->propg, propg1, sizeof(zd) * MM, hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->haj, haj, sizeof(zd) * 2 * NM, hipMemcpyHostToDevice, st));
+  if (c->mixed) {
+    RET(dalloc(c, &c->propg32, MM));
+    RET(launch(c, "convert", [&]() {
+      hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg, c->propg32, MM);
+    }));
+  }
+  // Spvn CSR to the device
+  const long nnz = spvn->nnz;
+  int* rp = nullptr;
+  int* col = nullptr;
+  zd* val = nullptr;
+  RET(dalloc(c, &rp, MM + 1));
+  RET(dalloc(c, &col, nnz));
+  RET(dalloc(c, &val, nnz));
+  {
+    std::vector<int> rp0(MM + 1);
+    for (long i = 0; i <= MM; i++) rp0[i] = spvn->rowptr[i] - spvn->rowptr[0];
+    HIPCHK(c, hipMemcpyAsync(rp, rp0.data(), sizeof(int) * (MM + 1), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(col, spvn->col, sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(val, spvn->val, sizeof(zd) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+  }
+  for (long p = 0; p < nnz; p += std::max<long>(1, nnz / 64))
+    if (spvn->col[p] < 0 || spvn->col[p] >= K) return fail(c, AFQMC_ERR_SHAPE, "Spvn column index out of range");
+  // dense Ld[n][ik] (FP64), then the half rotation Lt[s][n][a][k] = sum_i trial_s(i,a) Ld[n][i][k]
+  zd* Ld64 = nullptr;
+  RET(dalloc(c, &Ld64, K * MM));
+  HIPCHK(c, hipMemsetAsync(Ld64, 0, sizeof(zd) * K * MM, st));
+  RET(launch(c, "csr_densify", [&]() {
+    hipLaunchKernelGGL(csr_densify_kernel, dim3((unsigned)((MM + 3) / 4)), dim3(256), 0, st, (int)MM, rp, col, val,
+                       Ld64, MM);
+  }));
+  zd* Lt64 = nullptr;
+  RET(dalloc(c, &Lt64, 2 * K * NM));
+  for (int sp = 0; sp < 2; sp++) {
+    ZGemmArgs a = gargs();
+    a.A = c->trial + sp * MN;  // A(a, i) = trial[i][a]
+    a.sam = 1;
+    a.sak = N;
+    a.B = Ld64;  // B(k, i) = Ld[n][i][k], batch n
+    a.sbn = 1;
+    a.sbk = M;
+    a.sb1 = MM;
+    a.C = Lt64 + sp * K * NM;
+    a.scm = M;
+    a.scn = 1;
+    a.sc1 = NM;
+    a.m = (int)N;
+    a.n = (int)M;
+    a.k = (int)M;
+    RET((zgemm<CP_F64, double, double, double, false, false>(c, "halfrotate_gemm", a, (int)K)));
+  }
+  if (c->mode == AFQMC_FP64_CSR) {
+    c->sp_rp = rp;
+    c->sp_col = col;
+    c->sp_val = val;
+    c->sp_nnz = nnz;
+    // SpvnT CSR [K][2NM] with rotate.hpp's cutoff
+    int* cnt = nullptr;
+    RET(dalloc(c, &cnt, K));
+    RET(launch(c, "rot_count", [&]() {
+      hipLaunchKernelGGL(rot_count_kernel, dim3((unsigned)K), dim3(256), 0, st, Lt64, (int)K, NM, cutoff, cnt);
+    }));
+    std::vector<int> hc(K), hrp(K + 1, 0);
+    HIPCHK(c, hipMemcpyAsync(hc.data(), cnt, sizeof(int) * K, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    long tot = 0;
+    for (long i = 0; i < K; i++) {
+      hrp[i] = (int)tot;
+      tot += hc[i];
+      if (tot > 2147483647L) return fail(c, AFQMC_ERR_SHAPE, "SpvnT exceeds 2^31 nonzeros");
+    }
+    hrp[K] = (int)tot;
+    c->spt_nnz = tot;
+    RET(dalloc(c, &c->spt_rp, K + 1));
+    RET(dalloc(c, &c->spt_col, tot));
+    RET(dalloc(c, &c->spt_val, tot));
+    HIPCHK(c, hipMemcpyAsync(c->spt_rp, hrp.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, st));
+    RET(launch(c, "rot_fill", [&]() {
+      hipLaunchKernelGGL(rot_fill_kernel, dim3((unsigned)K), dim3(256), 0, st, Lt64, (int)K, NM, cutoff, c->spt_rp,
+                         c->spt_col, c->spt_val);
+    }));
+    HIPCHK(c, hipStreamSynchronize(st));
+    dfree(c, cnt);
+    dfree(c, Ld64);
+    if (vakbl && vakbl->nnz > 0) {
+      const long vn = vakbl->nnz, vr = vakbl->nrows;
+      RET(dalloc(c, &c->v_rp, vr + 1));
+      RET(dalloc(c, &c->v_col, vn));
+      RET(dalloc(c, &c->v_val, vn));
+      std::vector<int> rp0(vr + 1);
+      for (long i = 0; i <= vr; i++) rp0[i] = vakbl->rowptr[i] - vakbl->rowptr[0];
+      HIPCHK(c, hipMemcpyAsync(c->v_rp, rp0.data(), sizeof(int) * (vr + 1), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->v_col, vakbl->col, sizeof(int) * vn, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->v_val, vakbl->val, sizeof(zd) * vn, hipMemcpyHostToDevice, st));
+      RET(dalloc(c, &c->Gcloc, 2 * NM * c->nw));
+      HIPCHK(c, hipStreamSynchronize(st));
+      dfree(c, Lt64);
+    } else {
+      c->Lt = Lt64;  // Cholesky-form energy
+      RET(dalloc(c, &c->Gtmp, 2 * NM * c->nw));
+    }
+  } else {
+    dfree(c, rp);
+    dfree(c, col);
+    dfree(c, val);
+    if (c->mixed) {
+      zf* Ld32 = nullptr;
+      zf* Lt32 = nullptr;
+      RET(dalloc(c, &Ld32, K * MM));
+      RET(dalloc(c, &Lt32, 2 * K * NM));
+      RET(launch(c, "convert", [&]() {
+        hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, Ld32, K * MM);
+        hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64, Lt32,
+                           2 * K * NM);
+      }));
+      HIPCHK(c, hipStreamSynchronize(st));
+      dfree(c, Ld64);
+      dfree(c, Lt64);
+      c->Ld = Ld32;
+      c->Lt = Lt32;
+    } else {
+      c->Ld = Ld64;
+      c->Lt = Lt64;
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  c->have_system = true;
+  return AFQMC_OK;
+}
+
+int afqmc_get_spvnT(afqmc_ctx* c, long long* nnz, int* rowptr, int* col, double* val)
+{
+  if (!c || !nnz) return AFQMC_ERR_ARG;
+  if (c->mode != AFQMC_FP64_CSR || !c->spt_rp) return fail(c, AFQMC_ERR_STATE, "SpvnT exists in FP64_CSR mode only");
+  *nnz = c->spt_nnz;
+  if (!rowptr) return AFQMC_OK;
+  HIPCHK(c, hipMemcpyAsync(rowptr, c->spt_rp, sizeof(int) * (c->K + 1), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(col, c->spt_col, sizeof(int) * c->spt_nnz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(val, c->spt_val, sizeof(zd) * c->spt_nnz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_init_walkers(afqmc_ctx* c)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  if (!c->have_system) return fail(c, AFQMC_ERR_STATE, "upload a system first");
+  const long MN = (long)c->M * c->N;
+  long tot = std::max<long>((long)c->nw * 2 * MN, (long)c->nw * 8);
+  return launch(c, "init_walkers", [&]() {
+    hipLaunchKernelGGL(init_walkers_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->W, c->Wd,
+                       c->trial, c->nw, MN);
+  });
+}
+
+int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
+{
+  if (!c || !host) return AFQMC_ERR_ARG;
+  const long M = c->M, N = c->N, K = c->K, nw = c->nw;
+  const long MN = M * N, MM = M * M, NAK = 2 * MN;
+  hipStream_t st = c->stream;
+  auto raw = [&](void* dev, size_t bytes) -> int {
+    if (to_device) HIPCHK(c, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, st));
+    else HIPCHK(c, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    return AFQMC_OK;
+  };
+  // walker-last reference [R][nw] <-> walker-first device [nw][R] of type zd/zf
+  auto wlast = [&](void* dev, long R, bool is32) -> int {
+    std::vector<zd> tmp((size_t)R * nw);
+    std::vector<zf> t32;
+    if (to_device) {
+      htranspose((const zd*)host, tmp.data(), R, nw);
+      if (is32) {
+        t32.resize(tmp.size());
+        for (size_t i = 0; i < tmp.size(); i++) t32[i] = zf{(float)tmp[i].re, (float)tmp[i].im};
+        HIPCHK(c, hipMemcpyAsync(dev, t32.data(), sizeof(zf) * t32.size(), hipMemcpyHostToDevice, st));
+      } else {
+        HIPCHK(c, hipMemcpyAsync(dev, tmp.data(), sizeof(zd) * tmp.size(), hipMemcpyHostToDevice, st));
+      }
+      HIPCHK(c, hipStreamSynchronize(st));
+    } else {
+      if (is32) {
+        t32.resize(tmp.size());
+        HIPCHK(c, hipMemcpyAsync(t32.data(), dev, sizeof(zf) * t32.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        for (size_t i = 0; i < tmp.size(); i++) tmp[i] = zd{t32[i].re, t32[i].im};
+      } else {
+        HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, sizeof(zd) * tmp.size(), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+      }
+      htranspose(tmp.data(), (zd*)host, nw, R);
+    }
+    return AFQMC_OK;
+  };
+  const bool d = dense_layout(c);
+  switch (buffer) {
+  case AFQMC_BUF_W: return raw(c->W, sizeof(zd) * nw * 2 * MN);
+  case AFQMC_BUF_WDATA: return raw(c->Wd, sizeof(zd) * nw * 8);
+  case AFQMC_BUF_HYBRIDW: return raw(c->hybridW, sizeof(zd) * nw);
+  case AFQMC_BUF_G: return d ? wlast(c->G, NAK, c->mixed) : raw(c->G, sizeof(zd) * NAK * nw);
+  case AFQMC_BUF_VBIAS: return d ? wlast(c->vbias, K, false) : raw(c->vbias, sizeof(zd) * K * nw);
+  case AFQMC_BUF_X: return d ? wlast(c->X, K, c->mixed) : raw(c->X, sizeof(zd) * K * nw);
+  case AFQMC_BUF_VHS:
+    if (d) return wlast(c->vHS, MM, c->mixed);
+    RET(raw(c->vHS, sizeof(zd) * MM * nw));
+    if (to_device) {
+      dim3 grid((nw + 31) / 32, (unsigned)((MM + 31) / 32));
+      RET(launch(c, "vhs_transpose", [&]() {
+        hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, st, (const zd*)c->vHS, c->vHSw, (int)MM,
+                           (int)nw, nw, MM);
+      }));
+    }
+    return AFQMC_OK;
+  default: return fail(c, AFQMC_ERR_ARG, "unknown buffer");
+  }
+}
+
+#define NEED_SYSTEM(c)                                                                 \
+  do {                                                                                 \
+    if (!(c)) return AFQMC_ERR_ARG;                                                    \
+    if (!(c)->have_system) return fail((c), AFQMC_ERR_STATE, "upload a system first"); \
+  } while (0)
+
+int afqmc_mixed_density_matrix(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  RET(do_mdm(c));
+  return check_flags(c);
+}
+int afqmc_vbias(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  return do_vbias(c);
+}
+int afqmc_fields(afqmc_ctx* c, const double* x, long substep)
+{
+  NEED_SYSTEM(c);
+  const long KW = (long)c->K * c->nw;
+  const double* xdev = nullptr;
+  double* tmp = nullptr;
+  if (x) {
+    RET(dalloc(c, &tmp, KW));
+    HIPCHK(c, hipMemcpyAsync(tmp, x, sizeof(double) * KW, hipMemcpyHostToDevice, c->stream));
+    xdev = tmp;
+  } else {
+    if (!c->xstage || substep < 0 || substep >= c->nstage) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
+    xdev = c->xstage + substep * KW;
+  }
+  int r = do_fields(c, xdev);
+  if (r == AFQMC_OK) r = check_flags(c);
+  if (tmp) {
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c, tmp);
+  }
+  return r;
+}
+int afqmc_vhs(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  return do_vhs(c);
+}
+int afqmc_propagate(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  return do_propagate(c);
+}
+int afqmc_save_overlaps(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  return do_save_overlaps(c);
+}
+int afqmc_calculate_overlaps(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  RET(do_overlaps(c));
+  return check_flags(c);
+}
+int afqmc_weights(afqmc_ctx* c, double dt, double* eshift_inout)
+{
+  NEED_SYSTEM(c);
+  if (eshift_inout) HIPCHK(c, hipMemcpyAsync(c->eshift, eshift_inout, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  RET(do_weights(c, nullptr, dt));
+  if (eshift_inout) {
+    HIPCHK(c, hipMemcpyAsync(eshift_inout, c->eshift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return AFQMC_OK;
+}
+int afqmc_orthogonalize(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  return do_ortho(c);
+}
+int afqmc_energy(afqmc_ctx* c, double* eav_out)
+{
+  NEED_SYSTEM(c);
+  return do_energy(c, eav_out);
+}
+
+int afqmc_stage_fields(afqmc_ctx* c, const double* x_all, long nsub)
+{
+  if (!c || !x_all || nsub <= 0) return AFQMC_ERR_ARG;
+  const long KW = (long)c->K * c->nw;
+  if (c->xstage) {
+    (void)hipStreamSynchronize(c->stream);
+    dfree(c, c->xstage);
+    c->xstage = nullptr;
+    c->nstage = 0;
+  }
+  RET(dalloc(c, &c->xstage, KW * nsub));
+  HIPCHK(c, hipMemcpyAsync(c->xstage, x_all, sizeof(double) * KW * nsub, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->nstage = nsub;
+  return AFQMC_OK;
+}
+
+int afqmc_substep(afqmc_ctx* c, long substep, int do_ortho_, double dt)
+{
+  NEED_SYSTEM(c);
+  if (!c->xstage || substep < 0 || substep >= c->nstage) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
+  const long KW = (long)c->K * c->nw;
+  return substep_body(c, c->xstage + substep * KW, do_ortho_, dt);
+}
+
+int afqmc_get_eshift(afqmc_ctx* c, double* e)
+{
+  if (!c || !e) return AFQMC_ERR_ARG;
+  HIPCHK(c, hipMemcpyAsync(e, c->eshift, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+int afqmc_set_eshift(afqmc_ctx* c, double e)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  HIPCHK(c, hipMemcpyAsync(c->eshift, &e, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AFQMC_OK;
+}
+
+int afqmc_run(afqmc_ctx* c, long first, int nsteps, int nsubsteps, int northo, double dt, double* eav_out)
+{
+  NEED_SYSTEM(c);
+  if (nsteps < 0 || nsubsteps < 0 || northo <= 0) return AFQMC_ERR_ARG;
+  if (!c->xstage || first < 0 || first + (long)nsteps * nsubsteps > c->nstage)
+    return fail(c, AFQMC_ERR_STATE, "not enough staged fields");
+  long tot = first;
+  for (int step = 0; step < nsteps; step++) {
+    for (int sub = 0; sub < nsubsteps; sub++, tot++) {
+      int ortho = (tot > 0 && tot % northo == 0) ? 1 : 0;
+      RET(afqmc_substep(c, tot, ortho, dt));
+    }
+    if (eav_out) {
+      RET(do_mdm(c));
+      RET(do_energy(c, &eav_out[step]));
+    }
+  }
+  return check_flags(c);
+}
+
+int afqmc_use_graphs(afqmc_ctx* c, int enable)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  if (enable) return fail(c, AFQMC_ERR_UNSUPPORTED, "graph capture not built yet");
+  c->use_graphs = false;
+  return AFQMC_OK;
+}
+
+int afqmc_profile(afqmc_ctx* c, int enable)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  collect_events(c);
+  c->prof = enable != 0;
+  if (enable) c->stats.clear();
+  return AFQMC_OK;
+}
+
+int afqmc_kernel_stats(afqmc_ctx* c, int cap, char (*names)[64], long long* launches, double* total_ms, int* count)
+{
+  if (!c || !count) return AFQMC_ERR_ARG;
+  collect_events(c);
+  int i = 0;
+  for (auto& kv : c->stats) {
+    if (i < cap) {
+      if (names) {
+        std::strncpy(names[i], kv.first.c_str(), 63);
+        names[i][63] = 0;
+      }
+      if (launches) launches[i] = kv.second.n;
+      if (total_ms) total_ms[i] = kv.second.ms;
+    }
+    i++;
+  }
+  *count = i;
+  return AFQMC_OK;
+}
+
+} // extern "C"

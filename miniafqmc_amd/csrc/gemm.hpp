@@ -1,0 +1,279 @@
+// Complex GEMM on CDNA4 matrix cores.
+//
+//   C(m,n) = alpha * sum_k A(m,k) * B(n,k) + beta * C(m,n)      (batched)
+//
+// MFMA has no complex type.  The complex product is run as a real GEMM on the
+// interleaved storage: A is read as-is (real [m][2k]); B is expanded on the
+// fly to B'(2n+s, 2k+t) = {{b_r, -b_i}, {b_i, b_r}}[s][t] while its fragment
+// is built from LDS, so the real output C'(m, 2n+s) is the complex C in
+// interleaved storage.  No operand is ever stored expanded.
+//
+// Compute precisions:
+//   CP_F64    v_mfma_f64_16x16x4_f64, FP64 accumulation
+//   CP_F32    v_mfma_f32_16x16x4_f32, FP32 accumulation
+//   CP_BF16X3 v_mfma_f32_16x16x32_bf16 on a hi/lo split of both operands
+//             (x = hi + lo, hi = bf16(x), lo = bf16(x - hi)); the three
+//             products hi*hi + hi*lo + lo*hi accumulate in FP32.
+//
+// All 16x16 MFMAs here share one output layout: lane l holds rows
+// 4*(l/16)+j, j=0..3, of column l%16.  Within one MFMA, A and B use the same
+// lane->k assignment, so the k order inside a fragment is free; it is chosen
+// so that each lane reads whole complex numbers (FP64/FP32) or four
+// consecutive complex numbers (BF16) from LDS.
+#pragma once
+#include "common.hpp"
+#include <type_traits>
+
+enum { CP_F64 = 0, CP_F32 = 1, CP_BF16X3 = 2 };
+
+struct ZGemmArgs {
+  const void* A;
+  const void* B;
+  void* C;
+  long sam, sak;  // A(m,k) at A[m*sam + k*sak] (complex elements)
+  long sbn, sbk;  // B(n,k)
+  long scm, scn;  // C(m,n)
+  // batch b -> offset (b / bdiv) * s1 + (b % bdiv) * s2 (complex elements)
+  long sa1, sa2, sb1, sb2, sc1, sc2;
+  int bdiv;
+  int m, n, k;
+  double ar, ai, br, bi;
+  // optional second output with C's strides: D(m,n) = F(m,n) + alpha*AB
+  // (F may equal D; neither may be an operand of the same call).  C may be
+  // null when only D is wanted.
+  void* D;
+  const void* F;
+};
+
+template <int CP> struct LdsT;
+template <> struct LdsT<CP_F64> {
+  using T = zd;
+  static constexpr int PAD = 1;
+};
+template <> struct LdsT<CP_F32> {
+  using T = zf;
+  static constexpr int PAD = 2;
+};
+template <> struct LdsT<CP_BF16X3> {
+  using T = uint32_t;  // re | im << 16 (bf16 bits)
+  static constexpr int PAD = 4;
+};
+
+template <class Z> __device__ inline void zero_z(Z& z)
+{
+  z.re = 0;
+  z.im = 0;
+}
+
+template <int CP, class AT, class BT, class OT, bool AK, bool BK, int BM, int BNC, int BKC, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
+{
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM;          // real rows per wave
+  constexpr int WTN = 2 * BNC / WN;     // real cols per wave
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int EA = (BM * BKC + NT - 1) / NT;
+  constexpr int EB = (BNC * BKC + NT - 1) / NT;
+  static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile must be a multiple of 16");
+  static_assert(CP != CP_BF16X3 || BKC % 16 == 0, "bf16 k-tile must be a multiple of 16 complex");
+  static_assert(CP == CP_BF16X3 || BKC % 4 == 0, "k-tile must be a multiple of 4 complex");
+  using L = LdsT<CP>;
+  using LT = typename L::T;
+  constexpr int LDK = BKC + L::PAD;
+  constexpr int NPLANE = (CP == CP_BF16X3) ? 2 : 1;
+  __shared__ __align__(16) LT sA[NPLANE][BM * LDK];
+  __shared__ __align__(16) LT sB[NPLANE][BNC * LDK];
+  using AZ = typename ztype<AT>::type;
+  using BZ = typename ztype<BT>::type;
+  using ACC = typename std::conditional<CP == CP_F64, doublex4, floatx4>::type;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int g = lane >> 4, r = lane & 15, s = r & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BNC;
+  const int b = blockIdx.z;
+  const long bq = b / p.bdiv, br_ = b % p.bdiv;
+  const AZ* __restrict__ A = (const AZ*)p.A + bq * p.sa1 + br_ * p.sa2;
+  const BZ* __restrict__ B = (const BZ*)p.B + bq * p.sb1 + br_ * p.sb2;
+  const long coff = 2 * (bq * p.sc1 + br_ * p.sc2);
+  OT* C = p.C ? (OT*)p.C + coff : nullptr;
+  OT* D = p.D ? (OT*)p.D + coff : nullptr;
+  const OT* F = p.F ? (const OT*)p.F + coff : nullptr;
+
+  ACC acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) acc[i][j][q] = 0;
+
+  AZ ra[EA];
+  BZ rb[EB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < EA; i++) {
+      int e = tid + i * NT;
+      int row, kc;
+      if (AK) { row = e / BKC; kc = e % BKC; } else { kc = e / BM; row = e % BM; }
+      int gm = m0 + row, gk = k0 + kc;
+      if (e < BM * BKC && gm < p.m && gk < p.k) ra[i] = A[(long)gm * p.sam + (long)gk * p.sak];
+      else zero_z(ra[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < EB; i++) {
+      int e = tid + i * NT;
+      int col, kc;
+      if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
+      int gn = n0 + col, gk = k0 + kc;
+      if (e < BNC * BKC && gn < p.n && gk < p.k) rb[i] = B[(long)gn * p.sbn + (long)gk * p.sbk];
+      else zero_z(rb[i]);
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < EA; i++) {
+      int e = tid + i * NT;
+      if (e >= BM * BKC) break;
+      int row, kc;
+      if (AK) { row = e / BKC; kc = e % BKC; } else { kc = e / BM; row = e % BM; }
+      if constexpr (CP == CP_F64) {
+        sA[0][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
+      } else if constexpr (CP == CP_F32) {
+        sA[0][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
+      } else {
+        float xr = (float)ra[i].re, xi = (float)ra[i].im;
+        unsigned short hr = f2bf(xr), hi = f2bf(xi);
+        unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
+        sA[0][row * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
+        sA[1][row * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < EB; i++) {
+      int e = tid + i * NT;
+      if (e >= BNC * BKC) break;
+      int col, kc;
+      if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
+      if constexpr (CP == CP_F64) {
+        sB[0][col * LDK + kc] = zd{(double)rb[i].re, (double)rb[i].im};
+      } else if constexpr (CP == CP_F32) {
+        sB[0][col * LDK + kc] = zf{(float)rb[i].re, (float)rb[i].im};
+      } else {
+        float xr = (float)rb[i].re, xi = (float)rb[i].im;
+        unsigned short hr = f2bf(xr), hi = f2bf(xi);
+        unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
+        sB[0][col * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
+        sB[1][col * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+      }
+    }
+  };
+
+  const int nk = (p.k + BKC - 1) / BKC;
+  gload(0);
+  sstore();
+  __syncthreads();
+  const int arow0 = wm * WTM + r;           // + bi*16
+  const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
+  for (int kt = 0; kt < nk; kt++) {
+    if (kt + 1 < nk) gload((kt + 1) * BKC);
+    if constexpr (CP == CP_F64 || CP == CP_F32) {
+      using RT = typename std::conditional<CP == CP_F64, double, float>::type;
+#pragma unroll
+      for (int kq = 0; kq < BKC / 4; kq++) {
+        LT a[TM], bv[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[i] = sA[0][(arow0 + i * 16) * LDK + kq * 4 + g];
+#pragma unroll
+        for (int j = 0; j < TN; j++) bv[j] = sB[0][(bcol0 + j * 8) * LDK + kq * 4 + g];
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+          RT b0 = s ? bv[j].im : bv[j].re;
+          RT b1 = s ? bv[j].re : -bv[j].im;
+#pragma unroll
+          for (int i = 0; i < TM; i++) {
+            if constexpr (CP == CP_F64) {
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].re, b0, acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].im, b1, acc[i][j], 0, 0, 0);
+            } else {
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].re, b0, acc[i][j], 0, 0, 0);
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i].im, b1, acc[i][j], 0, 0, 0);
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (int kq = 0; kq < BKC / 16; kq++) {
+        uint4 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+          ah[i] = *(const uint4*)&sA[0][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
+          al[i] = *(const uint4*)&sA[1][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
+        }
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+          uint4 h = *(const uint4*)&sB[0][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
+          uint4 l = *(const uint4*)&sB[1][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
+          if (s == 0) {
+            h.x ^= 0x80000000u; h.y ^= 0x80000000u; h.z ^= 0x80000000u; h.w ^= 0x80000000u;
+            l.x ^= 0x80000000u; l.y ^= 0x80000000u; l.z ^= 0x80000000u; l.w ^= 0x80000000u;
+          } else {
+            h.x = (h.x >> 16) | (h.x << 16); h.y = (h.y >> 16) | (h.y << 16);
+            h.z = (h.z >> 16) | (h.z << 16); h.w = (h.w >> 16) | (h.w << 16);
+            l.x = (l.x >> 16) | (l.x << 16); l.y = (l.y >> 16) | (l.y << 16);
+            l.z = (l.z >> 16) | (l.z << 16); l.w = (l.w >> 16) | (l.w << 16);
+          }
+          bh[j] = h;
+          bl[j] = l;
+        }
+#pragma unroll
+        for (int i = 0; i < TM; i++) {
+          bf16x8 xah = __builtin_bit_cast(bf16x8, ah[i]);
+          bf16x8 xal = __builtin_bit_cast(bf16x8, al[i]);
+#pragma unroll
+          for (int j = 0; j < TN; j++) {
+            bf16x8 xbh = __builtin_bit_cast(bf16x8, bh[j]);
+            bf16x8 xbl = __builtin_bit_cast(bf16x8, bl[j]);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xah, xbh, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xah, xbl, acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xal, xbh, acc[i][j], 0, 0, 0);
+          }
+        }
+      }
+    }
+    if (kt + 1 < nk) {
+      __syncthreads();
+      sstore();
+      __syncthreads();
+    }
+  }
+
+  // epilogue: lanes (2c, 2c+1) hold Re and Im of complex column c
+  const bool has_beta = (p.br != 0.0 || p.bi != 0.0);
+#pragma unroll
+  for (int i = 0; i < TM; i++)
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        double v = (double)acc[i][j][q];
+        double pv = __shfl_xor(v, 1);
+        double re = s ? pv : v, im = s ? v : pv;
+        int row = m0 + wm * WTM + i * 16 + 4 * g + q;
+        int col = n0 + (wn * WTN + j * 16 + r) / 2;
+        if (row < p.m && col < p.n) {
+          long off = 2 * ((long)row * p.scm + (long)col * p.scn);
+          double o = s ? (p.ar * im + p.ai * re) : (p.ar * re - p.ai * im);
+          if (D) D[off + s] = (OT)((F ? (double)F[off + s] : 0.0) + o);
+          if (C) {
+            if (has_beta) {
+              double cr = (double)C[off], ci = (double)C[off + 1];
+              o += s ? (p.br * ci + p.bi * cr) : (p.br * cr - p.bi * ci);
+            }
+            C[off + s] = (OT)o;
+          }
+        }
+      }
+}

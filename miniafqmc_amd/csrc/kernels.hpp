@@ -1,0 +1,628 @@
+// Non-GEMM kernels of the walker step (FP64 unless stated).
+#pragma once
+#include "common.hpp"
+
+// ---------------------------------------------------------------------------
+// Batched in-place Gauss-Jordan inverse + determinant of N x N complex
+// matrices, FP64.  Follows LAPACK's pivoting on the column-major view used by
+// ma::invert / ma::determinant (src/Numerics/ma_operations.hpp:244-291,
+// ma_lapack.hpp:33-79): the matrix factorised is X = T^T, the pivot of step k
+// is the first row i >= k maximising |Re|+|Im| of X(i,k) (izamax), and
+// det = prod(pivots) * (-1)^(#swaps).  The Gauss-Jordan eliminations below
+// the diagonal are those of the LU factorisation, so pivots and determinant
+// match getrf; the inverse equals getri's up to rounding.
+//
+// Thread (i, cg) of the NMAX*CG threads holds row i, columns cg + CG*t of X in
+// registers.  Rows >= N are padded with the identity.
+// batch b: input T + b*mstride; det written to det[(b/2)*dq + (b%2)*dr].
+template <int NMAX, int CG>
+__global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
+                                                               zd* __restrict__ det, long dq, long dr, int N,
+                                                               long mstride, int* __restrict__ info)
+{
+  constexpr int NC = NMAX / CG;
+  __shared__ double red[NMAX];
+  __shared__ zd prow[NMAX], oldk[NMAX], colk[NMAX];
+  __shared__ int perm[NMAX], dest[NMAX];
+  const int tid = threadIdx.x;
+  const int i = tid / CG, cg = tid % CG;
+  const int lane = tid & 63;
+  const long b = blockIdx.x;
+  const zd* Tb = T + b * mstride;
+  zd x[NC];
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    int j = cg + CG * t;
+    if (i < N && j < N) x[t] = Tb[(long)j * N + i];  // X = T^T
+    else x[t] = zd{(i == j) ? 1.0 : 0.0, 0.0};
+  }
+  zd dt{1.0, 0.0};
+  bool singular = false;
+  for (int k = 0; k < N; k++) {
+    const int tk = k / CG, ck = k % CG;
+    // pivot search over column k, rows >= k
+    if (cg == ck) {
+      zd v{0, 0};
+#pragma unroll
+      for (int t = 0; t < NC; t++)
+        if (t == tk) v = x[t];
+      red[i] = (i >= k) ? zcabs1(v) : -1.0;
+    }
+    __syncthreads();
+    double bv = -2.0;
+    int bi = 0;
+    for (int q = lane; q < NMAX; q += 64) {
+      double v = red[q];
+      if (v > bv) { bv = v; bi = q; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      double ov = __shfl_xor(bv, off);
+      int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    const int p = bi;
+    if (i == p) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) prow[cg + CG * t] = x[t];
+    }
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) oldk[cg + CG * t] = x[t];
+    }
+    if (cg == ck) {
+      zd v{0, 0};
+#pragma unroll
+      for (int t = 0; t < NC; t++)
+        if (t == tk) v = x[t];
+      colk[i] = v;
+    }
+    if (tid == 0) perm[k] = p;
+    __syncthreads();
+    const zd piv = prow[k];
+    if (piv.re == 0.0 && piv.im == 0.0) {
+      singular = true;
+      dt = zd{0.0, 0.0};
+      break;
+    }
+    dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
+    const zd ipv = zdiv(zd{1.0, 0.0}, piv);
+    const bool swapped_in = (i == p && p != k);
+    const zd f = swapped_in ? colk[k] : colk[i];
+#pragma unroll
+    for (int t = 0; t < NC; t++) {
+      int j = cg + CG * t;
+      zd pr = (j == k) ? zd{1.0, 0.0} : prow[j];
+      zd sr = zmul(pr, ipv);  // scaled pivot row; column k holds 1/piv
+      if (i == k) {
+        x[t] = sr;
+      } else {
+        zd cur = swapped_in ? oldk[j] : x[t];
+        if (j == k) cur = zd{0.0, 0.0};
+        x[t] = zsub(cur, zmul(f, sr));
+      }
+    }
+  }
+  if (tid == 0) {
+    if (singular && info) atomicAdd(info, 1);
+    det[(b / 2) * dq + (b % 2) * dr] = dt;
+  }
+  if (!Tinv) return;
+  if (singular) {
+    // leave a defined value: zeros
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NC; t++) {
+      int j = cg + CG * t;
+      if (i < N && j < N) Tinv[b * mstride + (long)j * N + i] = zd{0, 0};
+    }
+    return;
+  }
+  // undo the row interchanges as column interchanges in reverse order
+  if (tid == 0) {
+    for (int c = 0; c < N; c++) dest[c] = c;  // dest used as colmap first
+    for (int k = N - 1; k >= 0; k--) {
+      int pk = perm[k];
+      int tmp = dest[k];
+      dest[k] = dest[pk];
+      dest[pk] = tmp;
+    }
+    for (int c = 0; c < N; c++) perm[dest[c]] = c;  // perm now = inverse map
+  }
+  __syncthreads();
+  // Z(:, c) = Y(:, colmap[c]) -> Y column j goes to column perm[j];
+  // T^-1[a][b] = Z[b][a]
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    int j = cg + CG * t;
+    if (i < N && j < N) Tinv[b * mstride + (long)perm[j] * N + i] = x[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CSR x dense, mySPBLAS::csrmm 'N' semantics (src/Numerics/sparse.hpp:166-180):
+// C(r, w) = alpha * sum_{p in row r, col<K} val[p] * B(col[p], w) + beta*C(r,w)
+// B(c, w) at B[c*ldb + w]; C(r, w) at C[r*ldc + w] (row-major, walker last).
+// One wave per (row, 64-walker chunk); nonzero indices are wave-uniform.
+__global__ __launch_bounds__(256) void csrmm_n_kernel(int nrows, int nw, int K, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, const zd* __restrict__ val,
+                                                      const zd* __restrict__ B, long ldb, zd* __restrict__ C,
+                                                      long ldc, zd alpha, zd beta)
+{
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int nchunk = (nw + 63) / 64;
+  const int r = wave / nchunk, ch = wave % nchunk;
+  if (r >= nrows) return;
+  const int w = ch * 64 + lane;
+  const int p0 = rowptr[0];
+  const int pb = __builtin_amdgcn_readfirstlane(rowptr[r] - p0);
+  const int pe = __builtin_amdgcn_readfirstlane(rowptr[r + 1] - p0);
+  zd acc{0, 0};
+  if (w < nw) {
+    for (int p = pb; p < pe; p++) {
+      int c = col[p];
+      if (c >= K) continue;
+      zd a = val[p];
+      zd bv = B[(long)c * ldb + w];
+      acc.re += a.re * bv.re - a.im * bv.im;
+      acc.im += a.re * bv.im + a.im * bv.re;
+    }
+    zd o = zmul(alpha, acc);
+    if (beta.re != 0.0 || beta.im != 0.0) o = zadd(o, zmul(beta, C[(long)r * ldc + w]));
+    C[(long)r * ldc + w] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled transpose of complex matrices: out[c][r] = in[r][c], in is R x Cc.
+// Output precision OT (double or float pairs).
+template <class IT, class OT>
+__global__ __launch_bounds__(256) void ztranspose_kernel(const IT* __restrict__ in, OT* __restrict__ out, int R,
+                                                         int Cc, long ldi, long ldo)
+{
+  __shared__ zd tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int yy = ty; yy < 32; yy += 8) {
+    int rr = by + yy, cc = bx + tx;
+    if (rr < R && cc < Cc) {
+      IT v = in[(long)rr * ldi + cc];
+      tile[yy][tx] = zd{(double)v.re, (double)v.im};
+    }
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    int cc = bx + yy, rr = by + tx;
+    if (rr < R && cc < Cc) {
+      zd v = tile[tx][yy];
+      OT o;
+      o.re = v.re;
+      o.im = v.im;
+      out[(long)cc * ldo + rr] = o;
+    }
+  }
+}
+
+// elementwise precision conversion / copy of n complex values
+template <class IT, class OT>
+__global__ void zconvert_kernel(const IT* __restrict__ in, OT* __restrict__ out, long n)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long st = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    IT v = in[i];
+    OT o;
+    o.re = v.re;
+    o.im = v.im;
+    out[i] = o;
+  }
+}
+
+// y += x (complex, FP64)
+__global__ void zaxpy1_kernel(const zd* __restrict__ x, zd* __restrict__ y, long n)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long st = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) y[i] = zadd(y[i], x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// X / hybrid weight (src/miniapps/miniafqmc_base.cpp:281-289).  Element (n, w)
+// of vbias at vb[n*vsn + w*vsw], of x at x[n*nw + w] (draws, [K][nw]), of X at
+// X[n*xsn + w*xsw].  hybridW[w] = -sum_n i*vb*(x + i/2*vb), X = x + i*vb.
+// One block per 64 walkers x (n split over 4 waves), deterministic reduction.
+template <class VT, class XT>
+__global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __restrict__ vb, long vsn, long vsw,
+                                                     const double* __restrict__ x, XT* __restrict__ X, long xsn,
+                                                     long xsw, zd* __restrict__ hybridW, int* __restrict__ nonfinite)
+{
+  __shared__ zd part[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane;
+  zd h{0, 0};
+  bool bad = false;
+  if (w < nw) {
+    for (int n = wv; n < K; n += 4) {
+      VT v = vb[(long)n * vsn + (long)w * vsw];
+      zd b{(double)v.re, (double)v.im};
+      double xr = x[(long)n * nw + w];
+      if (!isfinite(xr)) bad = true;
+      // t1 = i*vb ; t3 = x + i/2*vb ; hybridW -= t1*t3
+      zd t1{-b.im, b.re};
+      zd t3{xr - 0.5 * b.im, 0.5 * b.re};
+      zd t4 = zmul(t1, t3);
+      h.re -= t4.re;
+      h.im -= t4.im;
+      XT o;
+      o.re = xr - b.im;
+      o.im = b.re;
+      X[(long)n * xsn + (long)w * xsw] = o;
+    }
+  }
+  if (bad) atomicOr(nonfinite, 1);
+  part[wv][lane] = h;
+  __syncthreads();
+  if (wv == 0 && w < nw) {
+    zd t = part[0][lane];
+    for (int q = 1; q < 4; q++) t = zadd(t, part[q][lane]);
+    hybridW[w] = t;
+  }
+}
+
+// W_data[:,5] = W_data[:,4]; W_data[:,6..7] = W_data[:,2..3]
+__global__ void save_overlaps_kernel(zd* __restrict__ Wd, int nw)
+{
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  zd* d = Wd + (long)w * 8;
+  d[5] = d[4];
+  d[6] = d[2];
+  d[7] = d[3];
+}
+
+// weights / hybrid energy / Eshift (src/miniapps/miniafqmc_base.cpp:317-327).
+// One block of up to 1024 threads; fixed-order tree reduction for Eshift.
+__global__ __launch_bounds__(1024) void weights_kernel(zd* __restrict__ Wd, const zd* __restrict__ hybridW, int nw,
+                                                       double dt, double* __restrict__ eshift)
+{
+  __shared__ double red[1024];
+  const double Es = *eshift;
+  double et = 0.0;
+  for (int w = threadIdx.x; w < nw; w += blockDim.x) {
+    zd* d = Wd + (long)w * 8;
+    zd ratio = zdiv(zmul(d[2], d[3]), zmul(d[6], d[7]));
+    double ang = atan2(ratio.im, ratio.re);
+    double scale = fmax(0.0, cos(ang));
+    double lr = log(hypot(ratio.re, ratio.im));
+    zd h = hybridW[w];
+    d[4] = zd{-(h.re + lr) / dt, -(h.im + ang) / dt};
+    double f = scale * exp(-dt * (0.5 * (d[4].re + d[5].re) - Es));
+    d[1] = zd{d[1].re * f, d[1].im * f};
+    et += d[4].re;
+  }
+  red[threadIdx.x] = et;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *eshift = red[0] / nw;
+}
+
+// W = conj(trial) per spin, W_data row = (0, 1, 0, ...)
+__global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, const zd* __restrict__ trial, int nw,
+                                    long MN)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long tot = (long)nw * 2 * MN;
+  if (i < tot) {
+    long rem = i % (2 * MN);
+    W[i] = zconj(trial[rem]);
+  }
+  if (i < (long)nw * 8) Wd[i] = zd{(i % 8 == 1) ? 1.0 : 0.0, 0.0};
+}
+
+// ---------------------------------------------------------------------------
+// Householder LQ with LAPACK conventions on the column-major view A = W^T
+// (N x M) of each W[w][s] ([M][N] row-major): zgelq2 followed by zungl2
+// (ma::gelqf / ma::glq, src/Numerics/ma_lapack.hpp:134-221; called by
+// afqmc_sys::orthogonalize, src/AFQMC/afqmc_sys.hpp:175-207).  A(i, c) is
+// W[c][i]: row i of A is column i of W.  One block per matrix; reflectors
+// applied with a block-wide GEMV + rank-1 update through L2.
+__device__ inline double block_sum(double v, double* sh)
+{
+  // 256 threads
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double t = sh[0] + sh[1] + sh[2] + sh[3];
+  return t;
+}
+
+__global__ __launch_bounds__(256) void lq_kernel(zd* __restrict__ Wall, int M, int N)
+{
+  constexpr int NTH = 256;
+  extern __shared__ zd tau[];  // [N]
+  __shared__ zd wsum[2 * 128]; // partial reductions of w = A v over two column halves
+  __shared__ double sh[8];
+  zd* W = Wall + (long)blockIdx.x * M * N;
+  const int tid = threadIdx.x;
+#define AIC(i, c) W[(long)(c) * N + (i)]
+  // ---- zgelq2 ----
+  for (int i = 0; i < N; i++) {
+    // conj row i (c >= i) and norm of A(i, i+1:M)
+    double ss = 0.0;
+    for (int c = i + tid; c < M; c += NTH) {
+      zd v = zconj(AIC(i, c));
+      AIC(i, c) = v;
+      if (c > i) ss += v.re * v.re + v.im * v.im;
+    }
+    double xn2 = block_sum(ss, sh);
+    __syncthreads();
+    zd alpha = AIC(i, i);
+    double xnorm = sqrt(xn2);
+    zd taui{0, 0}, scal{0, 0};
+    double beta = 0.0;
+    bool apply = !(xnorm == 0.0 && alpha.im == 0.0);
+    if (apply) {
+      double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+      beta = -copysign(nrm, alpha.re);  // LAPACK: -sign(|(alpha,x)|, Re alpha)
+      taui = zd{(beta - alpha.re) / beta, -alpha.im / beta};
+      scal = zdiv(zd{1.0, 0.0}, zd{alpha.re - beta, alpha.im});
+    }
+    if (tid == 0) tau[i] = taui;
+    if (apply) {
+      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zmul(scal, AIC(i, c));
+    }
+    __syncthreads();
+    // apply H(i) from the right to rows r = i+1..N-1 over columns c = i..M-1 with v = [1, A(i,i+1:)]
+    if (apply && i < N - 1) {
+      const int nr = N - i - 1;
+      // w_r = sum_c A(r,c) v_c ; threads: r fastest, c split in NTH/nr... use 2D loop
+      for (int r0 = 0; r0 < nr; r0 += 128) {
+        const int rr = tid & 127, cp = tid >> 7;  // 2 column parts
+        const int r = i + 1 + r0 + rr;
+        zd acc{0, 0};
+        if (r0 + rr < nr) {
+          for (int c = i + cp; c < M; c += 2) {
+            zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
+            acc = zadd(acc, zmul(AIC(r, c), v));
+          }
+        }
+        wsum[cp * 128 + rr] = acc;
+        __syncthreads();
+        if (tid < 128 && r0 + tid < nr) wsum[tid] = zadd(wsum[tid], wsum[128 + tid]);
+        __syncthreads();
+        // A(r,c) -= tau * w_r * conj(v_c)
+        for (int e = tid; e < 128 * (M - i); e += NTH) {
+          int rl = e % 128, c = i + e / 128;
+          if (r0 + rl >= nr) continue;
+          int rA = i + 1 + r0 + rl;
+          zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
+          zd t = zmul(zmul(taui, wsum[rl]), zconj(v));
+          AIC(rA, c) = zsub(AIC(rA, c), t);
+        }
+        __syncthreads();
+      }
+    }
+    if (tid == 0) AIC(i, i) = zd{beta, 0.0};
+    if (!apply && tid == 0) AIC(i, i) = alpha;
+    __syncthreads();
+    for (int c = i + tid; c < M; c += NTH) AIC(i, c) = zconj(AIC(i, c));
+    __syncthreads();
+  }
+  // ---- zungl2 (k = N = m rows) ----
+  for (int i = N - 1; i >= 0; i--) {
+    const zd ti = tau[i];
+    if (i < M - 1) {
+      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zconj(AIC(i, c));
+      __syncthreads();
+      if (i < N - 1) {
+        const zd tc = zconj(ti);
+        const int nr = N - i - 1;
+        const bool nz = !(tc.re == 0.0 && tc.im == 0.0);
+        for (int r0 = 0; r0 < nr && nz; r0 += 128) {
+          const int rr = tid & 127, cp = tid >> 7;
+          const int r = i + 1 + r0 + rr;
+          zd acc{0, 0};
+          if (r0 + rr < nr) {
+            for (int c = i + cp; c < M; c += 2) {
+              zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
+              acc = zadd(acc, zmul(AIC(r, c), v));
+            }
+          }
+          wsum[cp * 128 + rr] = acc;
+          __syncthreads();
+          if (tid < 128 && r0 + tid < nr) wsum[tid] = zadd(wsum[tid], wsum[128 + tid]);
+          __syncthreads();
+          for (int e = tid; e < 128 * (M - i); e += NTH) {
+            int rl = e % 128, c = i + e / 128;
+            if (r0 + rl >= nr) continue;
+            int rA = i + 1 + r0 + rl;
+            zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
+            zd t = zmul(zmul(tc, wsum[rl]), zconj(v));
+            AIC(rA, c) = zsub(AIC(rA, c), t);
+          }
+          __syncthreads();
+        }
+      }
+      const zd mt{-ti.re, -ti.im};
+      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zconj(zmul(mt, AIC(i, c)));
+    }
+    __syncthreads();
+    if (tid == 0) AIC(i, i) = zd{1.0 - ti.re, ti.im};
+    for (int c = tid; c < i; c += NTH) AIC(i, c) = zd{0, 0};
+    __syncthreads();
+  }
+#undef AIC
+}
+
+// ---------------------------------------------------------------------------
+// Per-walker complex dot products without conjugation, reduction over i:
+//   out[w] = scale * sum_i A(i, w) * B(i, w) (+ out[w] if accumulate)
+// A(i,w) at A[i*asi + w*asw], same for B.  One block per walker.
+template <class AT, class BT>
+__global__ __launch_bounds__(256) void coldot_kernel(int n, const AT* __restrict__ A, long asi, long asw,
+                                                     const BT* __restrict__ B, long bsi, long bsw, zd* out, long osw,
+                                                     double scale, int accumulate)
+{
+  __shared__ zd sh[256];
+  const int w = blockIdx.x;
+  zd acc{0, 0};
+  for (int i = threadIdx.x; i < n; i += 256) {
+    AT a = A[(long)i * asi + (long)w * asw];
+    BT bb = B[(long)i * bsi + (long)w * bsw];
+    acc.re += (double)a.re * (double)bb.re - (double)a.im * (double)bb.im;
+    acc.im += (double)a.re * (double)bb.im + (double)a.im * (double)bb.re;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = zadd(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    zd r{sh[0].re * scale, sh[0].im * scale};
+    if (accumulate) r = zadd(r, out[(long)w * osw]);
+    out[(long)w * osw] = r;
+  }
+}
+
+// Exchange reduction: for each batch item q (a walker-spin and a chunk of
+// Cholesky vectors), sum over n in the chunk of sum_ab Y_n(a,b) Y_n(b,a),
+// with Y stored [nq][N][N] per n.  out[q] += result (FP64 accumulation).
+template <class YT>
+__global__ __launch_bounds__(256) void trace_sq_kernel(const YT* __restrict__ Y, int N, int nq, long ystride,
+                                                       zd* __restrict__ out, long ostride, int nbatch)
+{
+  __shared__ zd sh[256];
+  const int q = blockIdx.x;  // batch item
+  const YT* Yq = Y + (long)q * ystride;
+  zd acc{0, 0};
+  const long NN = (long)N * N;
+  for (long e = threadIdx.x; e < (long)nq * NN; e += 256) {
+    long n = e / NN, rem = e % NN;
+    int a = (int)(rem / N), b = (int)(rem % N);
+    YT y1 = Yq[n * NN + (long)a * N + b];
+    YT y2 = Yq[n * NN + (long)b * N + a];
+    acc.re += (double)y1.re * (double)y2.re - (double)y1.im * (double)y2.im;
+    acc.im += (double)y1.re * (double)y2.im + (double)y1.im * (double)y2.re;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = zadd(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(long)q * ostride] = zadd(out[(long)q * ostride], sh[0]);
+}
+
+// direct term: out[w] += sum_n v(n,w)^2 where v(n,w) at v[n*vsn + w*vsw]
+template <class VT>
+__global__ __launch_bounds__(256) void sumsq_kernel(int K, const VT* __restrict__ v, long vsn, long vsw,
+                                                    zd* __restrict__ out, double scale)
+{
+  __shared__ zd sh[256];
+  const int w = blockIdx.x;
+  zd acc{0, 0};
+  for (int n = threadIdx.x; n < K; n += 256) {
+    VT x = v[(long)n * vsn + (long)w * vsw];
+    acc.re += (double)x.re * (double)x.re - (double)x.im * (double)x.im;
+    acc.im += 2.0 * (double)x.re * (double)x.im;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = zadd(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[w].re += scale * sh[0].re;
+    out[w].im += scale * sh[0].im;
+  }
+}
+
+// E_L assembly for the Cholesky form:
+// Wd[w][0] = 0.5*(direct[w] - exch[w]) + e1[w]
+__global__ void energy_assemble_kernel(zd* __restrict__ Wd, const zd* __restrict__ direct,
+                                       const zd* __restrict__ exch, const zd* __restrict__ e1, int nw)
+{
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  zd e2{0.5 * (direct[w].re - exch[w].re), 0.5 * (direct[w].im - exch[w].im)};
+  Wd[(long)w * 8] = zadd(e2, e1[w]);
+}
+
+// ---------------------------------------------------------------------------
+// Setup kernels.
+// Scatter CSR Spvn [M*M][K] into dense Ld[n][ik] (Cholesky-vector major).
+__global__ void csr_densify_kernel(int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                   const zd* __restrict__ val, zd* __restrict__ Ld, long ldn)
+{
+  int r = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (r >= nrows) return;
+  int p0 = rowptr[0];
+  for (int p = rowptr[r] - p0 + lane; p < rowptr[r + 1] - p0; p += 64) Ld[(long)col[p] * ldn + r] = val[p];
+}
+
+// per-row count of |Lt(n, j)| > cutoff, Lt spin-major [2][K][NM]: row n of
+// SpvnT is (Lt[0][n][:], Lt[1][n][:]).
+__global__ void rot_count_kernel(const zd* __restrict__ Lt, int K, long NM, double cutoff, int* __restrict__ cnt)
+{
+  __shared__ int sh[256];
+  const int n = blockIdx.x;
+  int c = 0;
+  for (long j = threadIdx.x; j < 2 * NM; j += 256) {
+    int sp = (int)(j / NM);
+    zd v = Lt[(long)sp * K * NM + (long)n * NM + (j % NM)];
+    if (hypot(v.re, v.im) > cutoff) c++;
+  }
+  sh[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cnt[n] = sh[0];
+}
+
+// fill of the CSR rows in ascending column order: one block per row, a
+// block-wide prefix over chunks of 256 columns.
+__global__ void rot_fill_kernel(const zd* __restrict__ Lt, int K, long NM, double cutoff,
+                                const int* __restrict__ rowptr, int* __restrict__ colo, zd* __restrict__ valo)
+{
+  __shared__ int sh[256];
+  const int n = blockIdx.x;
+  int base = rowptr[n];
+  for (long j0 = 0; j0 < 2 * NM; j0 += 256) {
+    long j = j0 + threadIdx.x;
+    bool keep = false;
+    zd v{0, 0};
+    if (j < 2 * NM) {
+      int sp = (int)(j / NM);
+      v = Lt[(long)sp * K * NM + (long)n * NM + (j % NM)];
+      keep = hypot(v.re, v.im) > cutoff;
+    }
+    sh[threadIdx.x] = keep ? 1 : 0;
+    __syncthreads();
+    // inclusive scan (Hillis-Steele)
+    for (int o = 1; o < 256; o <<= 1) {
+      int t = (threadIdx.x >= o) ? sh[threadIdx.x - o] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (keep) {
+      int pos = base + sh[threadIdx.x] - 1;
+      colo[pos] = (int)j;
+      valo[pos] = v;
+    }
+    int tot = sh[255];
+    __syncthreads();
+    base += tot;
+  }
+}
