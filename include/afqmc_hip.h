@@ -133,6 +133,15 @@ int afqmc_orthogonalize(afqmc_ctx* ctx);
  * does (miniafqmc_base.cpp:342-343). */
 int afqmc_energy(afqmc_ctx* ctx, double* eav_out);
 
+/* ---- numerics seam ----------------------------------------------------------- */
+/* Complex GEMM on the GPU with ma::product's row-major meaning
+ * (src/Numerics/ma_operations.hpp:94-107 -> BLAS::gemm,
+ * src/Numerics/OhmmsBlas.h:270-278): C[m][n] = alpha*op(A)*op(B) + beta*C,
+ * op in {'N','T'}, host arrays.  precision: 0 FP64, 1 FP32, 2 split-BF16
+ * (the arithmetic of the modes above). */
+int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int k, const double* alpha, const double* A,
+                int lda, const double* B, int ldb, const double* beta, double* C, int ldc);
+
 /* ---- whole-loop entry points -------------------------------------------- */
 /* Device copy of the real N(0,1) draws of `nsub` substeps, x_all
  * [nsub][K][nwalk]; afqmc_fields(ctx, NULL, s) and afqmc_run use them. */

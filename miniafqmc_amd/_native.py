@@ -34,7 +34,7 @@ EXPORTED = [
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
     "afqmc_weights", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
     "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run",
-    "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats",
+    "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats", "afqmc_zgemm",
     "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
 ]
 
@@ -115,6 +115,7 @@ def _load():
         "afqmc_use_graphs": (I, [P, I]),
         "afqmc_profile": (I, [P, I]),
         "afqmc_kernel_stats": (I, [P, I, P, C.POINTER(C.c_longlong), D, C.POINTER(C.c_int)]),
+        "afqmc_zgemm": (I, [I, C.c_char, C.c_char, I, I, I, D, D, I, D, I, D, D, I]),
         "afqmc_synth_create": (I, [C.POINTER(SynthParams), C.POINTER(P)]),
         "afqmc_synth_destroy": (I, [P]),
         "afqmc_synth_get": (I, [P, C.POINTER(SynthView)]),

@@ -88,6 +88,7 @@ struct afqmc_ctx {
   // energy scratch
   zd* e_direct = nullptr;
   zd* e_exch = nullptr;
+  zd* e_exch_part = nullptr;  // [nw][2]
   zd* e_one = nullptr;
   zd* e_v = nullptr;      // [nw][K] direct-term contractions
   void* Ybuf = nullptr;
@@ -546,7 +547,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
   const double inv_dt = 1.0 / c->dt_sys;
   RET(launch(c, "energy_zero", [&]() {
     (void)hipMemsetAsync(c->e_direct, 0, sizeof(zd) * nw, c->stream);
-    (void)hipMemsetAsync(c->e_exch, 0, sizeof(zd) * nw, c->stream);
+    (void)hipMemsetAsync(c->e_exch_part, 0, sizeof(zd) * 2 * nw, c->stream);
   }));
   // direct term: v(w,n) = sum Lt G ; direct = sum_n v^2 / dt
   RET((dense_contract_vbias<CP, GT, LT_>(c, Gp, c->e_v, "energy_v_gemm")));
@@ -554,35 +555,43 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
     hipLaunchKernelGGL((sumsq_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, c->e_v, 1L, (long)K, c->e_direct,
                        inv_dt);
   }));
-  // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba; vectors in chunks
-  const long per_n = (long)N * N;
-  int nq = (int)std::min<long>(K, std::max<long>(1, c->ybuf_elems / per_n));
-  for (int w = 0; w < nw; w++) {
-    for (int sp = 0; sp < 2; sp++) {
-      for (int n0 = 0; n0 < K; n0 += nq) {
-        int cnt = std::min(nq, K - n0);
-        ZGemmArgs a = gargs();
-        a.A = (const LT_*)c->Lt + 2 * ((long)sp * K * NM + (long)n0 * NM);
-        a.sam = M;
-        a.sak = 1;
-        a.B = (const GT*)Gp + 2 * ((long)w * 2 * NM + sp * NM);
-        a.sbn = M;
-        a.sbk = 1;
-        a.C = c->Ybuf;
-        a.scm = N;
-        a.scn = 1;
-        a.m = cnt * N;
-        a.n = N;
-        a.k = M;
-        a.ar = std::sqrt(inv_dt);  // unscale both factors: Y = Y_scaled / sqrt(dt)... applied once to Y
-        RET((zgemm<CP, LT_, GT, YT, true, true>(c, "energy_y_gemm", a, 1)));
-        RET(launch(c, "energy_trace", [&]() {
-          hipLaunchKernelGGL((trace_sq_kernel<typename ztype<YT>::type>), dim3(1), dim3(256), 0, c->stream,
-                             (const typename ztype<YT>::type*)c->Ybuf, N, cnt, 0L, c->e_exch + w, 0L, 1);
-        }));
-      }
-    }
+  // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba, for all
+  // walker-spin pairs at once over chunks of nq vectors
+  const long per_item = (long)N * N;
+  const int nq = (int)std::max<long>(1, std::min<long>(K, c->ybuf_elems / (2L * nw * per_item)));
+  for (int n0 = 0; n0 < K; n0 += nq) {
+    const int cnt = std::min(nq, K - n0);
+    ZGemmArgs a = gargs();
+    a.A = (const LT_*)c->Lt + 2 * ((long)n0 * NM);
+    a.sam = M;
+    a.sak = 1;
+    a.sa1 = 0;
+    a.sa2 = (long)K * NM;  // spin block of Lt
+    a.B = Gp;              // G[w][s]
+    a.sbn = M;
+    a.sbk = 1;
+    a.sb1 = 2 * NM;
+    a.sb2 = NM;
+    a.bdiv = 2;
+    a.C = c->Ybuf;
+    a.scm = N;
+    a.scn = 1;
+    a.sc1 = 2L * cnt * per_item;
+    a.sc2 = (long)cnt * per_item;
+    a.m = cnt * N;
+    a.n = N;
+    a.k = M;
+    a.ar = std::sqrt(inv_dt);  // Y of the unscaled vectors
+    RET((zgemm<CP, LT_, GT, YT, true, true>(c, "energy_y_gemm", a, 2 * nw)));
+    RET(launch(c, "energy_trace", [&]() {
+      hipLaunchKernelGGL((trace_sq_kernel<typename ztype<YT>::type>), dim3(2 * nw), dim3(256), 0, c->stream,
+                         (const typename ztype<YT>::type*)c->Ybuf, N, cnt, (long)cnt * per_item, c->e_exch_part, 1L,
+                         2 * nw);
+    }));
   }
+  RET(launch(c, "energy_pairsum", [&]() {
+    hipLaunchKernelGGL(pairsum_kernel, dim3((nw + 255) / 256), dim3(256), 0, c->stream, c->e_exch_part, c->e_exch, nw);
+  }));
   // one-body: e1 = sum_i G(w,i) haj(i)
   RET(launch(c, "energy_one", [&]() {
     hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, zd>), dim3(nw), dim3(256), 0, c->stream, (int)(2 * NM),
@@ -734,9 +743,10 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   chk(dalloc(c, &c->flags, 4));
   chk(dalloc(c, &c->e_direct, nw));
   chk(dalloc(c, &c->e_exch, nw));
+  chk(dalloc(c, &c->e_exch_part, 2 * nw));
   chk(dalloc(c, &c->e_one, nw));
   chk(dalloc(c, &c->e_v, nw * K));
-  c->ybuf_elems = std::max<long>(N * N, std::min<long>(K * N * N, 1L << 24));
+  c->ybuf_elems = std::max<long>(2 * nw * N * N, std::min<long>(2 * nw * K * N * N, 1L << 25));
   chk(dalloc_bytes(c, &c->Ybuf, c->ybuf_elems * zs));
   if (r != AFQMC_OK) {
     afqmc_ctx_destroy(c);
@@ -1197,3 +1207,75 @@ int afqmc_kernel_stats(afqmc_ctx* c, int cap, char (*names)[64], long long* laun
 }
 
 } // extern "C"
+
+namespace {
+template <int CP, bool AK, bool BK> int zgemm_host(afqmc_ctx* c, ZGemmArgs a)
+{
+  return zgemm<CP, double, double, double, AK, BK>(c, "zgemm", a, 1);
+}
+template <int CP> int zgemm_ops(afqmc_ctx* c, ZGemmArgs a, bool ak, bool bk)
+{
+  if (ak && bk) return zgemm_host<CP, true, true>(c, a);
+  if (ak && !bk) return zgemm_host<CP, true, false>(c, a);
+  if (!ak && bk) return zgemm_host<CP, false, true>(c, a);
+  return zgemm_host<CP, false, false>(c, a);
+}
+} // namespace
+
+extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int k, const double* alpha,
+                           const double* A, int lda, const double* B, int ldb, const double* beta, double* Cm,
+                           int ldc)
+{
+  if (!alpha || !beta || !A || !B || !Cm || m < 0 || n < 0 || k < 0) return AFQMC_ERR_ARG;
+  if ((opA != 'N' && opA != 'T') || (opB != 'N' && opB != 'T') || precision < 0 || precision > 2)
+    return AFQMC_ERR_ARG;
+  const long ar = (opA == 'N') ? m : k, ac = (opA == 'N') ? k : m;
+  const long br = (opB == 'N') ? k : n, bc = (opB == 'N') ? n : k;
+  if (lda < ac || ldb < bc || ldc < n) return AFQMC_ERR_SHAPE;
+  if (m == 0 || n == 0) return AFQMC_OK;
+  afqmc_ctx tmp;
+  afqmc_ctx* c = &tmp;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return AFQMC_ERR_HIP;
+  zd *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  int r = AFQMC_OK;
+  auto run = [&]() -> int {
+    RET(dalloc(c, &dA, ar * lda));
+    RET(dalloc(c, &dB, br * ldb));
+    RET(dalloc(c, &dC, (long)m * ldc));
+    HIPCHK(c, hipMemcpyAsync(dA, A, sizeof(zd) * ar * lda, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dB, B, sizeof(zd) * br * ldb, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dC, Cm, sizeof(zd) * m * ldc, hipMemcpyHostToDevice, c->stream));
+    ZGemmArgs a = gargs();
+    a.A = dA;
+    a.B = dB;
+    a.C = dC;
+    const bool ak = (opA == 'N'), bk = (opB == 'T');
+    a.sam = ak ? lda : 1;
+    a.sak = ak ? 1 : lda;
+    a.sbn = bk ? ldb : 1;
+    a.sbk = bk ? 1 : ldb;
+    a.scm = ldc;
+    a.scn = 1;
+    a.m = m;
+    a.n = n;
+    a.k = k;
+    a.ar = alpha[0];
+    a.ai = alpha[1];
+    a.br = beta[0];
+    a.bi = beta[1];
+    if (k == 0) a.ar = a.ai = 0.0;
+    if (precision == 0) RET(zgemm_ops<CP_F64>(c, a, ak, bk));
+    else if (precision == 1) RET(zgemm_ops<CP_F32>(c, a, ak, bk));
+    else RET(zgemm_ops<CP_BF16X3>(c, a, ak, bk));
+    HIPCHK(c, hipMemcpyAsync(Cm, dC, sizeof(zd) * m * ldc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AFQMC_OK;
+  };
+  r = run();
+  (void)hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+  return r;
+}

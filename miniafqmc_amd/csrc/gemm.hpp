@@ -15,8 +15,9 @@
 //             (x = hi + lo, hi = bf16(x), lo = bf16(x - hi)); the three
 //             products hi*hi + hi*lo + lo*hi accumulate in FP32.
 //
-// All 16x16 MFMAs here share one output layout: lane l holds rows
-// 4*(l/16)+j, j=0..3, of column l%16.  Within one MFMA, A and B use the same
+// Output layouts (lane l, accumulator register j, column l%16): the FP32 and
+// BF16 16x16 forms hold rows 4*(l/16)+j; the FP64 form holds rows
+// 4*j+(l/16).  Within one MFMA, A and B use the same
 // lane->k assignment, so the k order inside a fragment is free; it is chosen
 // so that each lane reads whole complex numbers (FP64/FP32) or four
 // consecutive complex numbers (BF16) from LDS.
@@ -261,7 +262,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         double v = (double)acc[i][j][q];
         double pv = __shfl_xor(v, 1);
         double re = s ? pv : v, im = s ? v : pv;
-        int row = m0 + wm * WTM + i * 16 + 4 * g + q;
+        // FP64 16x16x4 interleaves rows (row = 4*reg + lane/16); the FP32 and
+        // BF16 16x16 forms keep 4 consecutive rows per lane group.
+        int row = m0 + wm * WTM + i * 16 + ((CP == CP_F64) ? (4 * q + g) : (4 * g + q));
         int col = n0 + (wn * WTN + j * 16 + r) / 2;
         if (row < p.m && col < p.n) {
           long off = 2 * ((long)row * p.scm + (long)col * p.scn);

@@ -520,6 +520,13 @@ __global__ __launch_bounds__(256) void trace_sq_kernel(const YT* __restrict__ Y,
   if (threadIdx.x == 0) out[(long)q * ostride] = zadd(out[(long)q * ostride], sh[0]);
 }
 
+// out[w] = part[2w] + part[2w+1] (the two spins)
+__global__ void pairsum_kernel(const zd* __restrict__ part, zd* __restrict__ out, int nw)
+{
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < nw) out[w] = zadd(part[2 * w], part[2 * w + 1]);
+}
+
 // direct term: out[w] += sum_n v(n,w)^2 where v(n,w) at v[n*vsn + w*vsw]
 template <class VT>
 __global__ __launch_bounds__(256) void sumsq_kernel(int K, const VT* __restrict__ v, long vsn, long vsw,

@@ -866,6 +866,7 @@ int orc_run(int M, int N, int K, int nw, int nsteps, int nsubsteps, int northo, 
       hybridW(nw);
   double tsec[11] = {0};
   auto energy = [&]() {
+    if (energy_mode == 2) return 0.0;  // timing samples: no energy evaluation
     if (energy_mode == 0) return sys_energy_vakbl(M, N, nw, Wd, Gc.data(), CCZ(haj), NAK, v_rp, v_col, CCZ(v_val));
     return sys_energy_chol(M, N, K, nw, Wd, Gc.data(), CCZ(haj), CCZ(Lt));
   };

@@ -1,0 +1,58 @@
+"""The C-ABI library loads without a GPU and exports every declared symbol."""
+import os
+import re
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "afqmc_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(afqmc_\w+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    import miniafqmc_amd._native as nat
+    syms = declared_symbols()
+    assert len(syms) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", nat.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (afqmc_\w+)", out))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    assert sorted(nat.EXPORTED) == syms
+    for s in syms:
+        getattr(nat.lib, s)
+
+
+def test_ctx_create_rejects_bad_shapes_without_gpu():
+    import ctypes as C
+    import miniafqmc_amd._native as nat
+    h = C.c_void_p()
+    assert nat.lib.afqmc_ctx_create(0, 4, 8, 2, 0, C.byref(h)) == 2   # AFQMC_ERR_SHAPE
+    assert nat.lib.afqmc_ctx_create(8, 9, 8, 2, 0, C.byref(h)) == 2   # N > M
+    assert nat.lib.afqmc_ctx_create(8, 4, 8, 2, 7, C.byref(h)) == 1   # bad mode
+    assert nat.lib.afqmc_ctx_create(300, 200, 8, 2, 0, C.byref(h)) == 8  # N > 128 unsupported
+    assert nat.lib.afqmc_ctx_create(8, 4, 8, 2, 0, None) == 1
+
+
+def test_synth_rejects_bad_params():
+    import ctypes as C
+    import miniafqmc_amd._native as nat
+    h = C.c_void_p()
+    p = nat.SynthParams(8, 10, 4, 0.5, 0.1, 1.0, 0.01, 1, 0)
+    assert nat.lib.afqmc_synth_create(C.byref(p), C.byref(h)) == 2
+    p = nat.SynthParams(8, 4, 4, 0.0, 0.1, 1.0, 0.01, 1, 0)
+    assert nat.lib.afqmc_synth_create(C.byref(p), C.byref(h)) == 2
+
+
+def test_product_imports_no_oracle():
+    """The product package never references the oracle."""
+    pkg = os.path.join(ROOT, "miniafqmc_amd")
+    for dp, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")) or f == "Makefile":
+                txt = open(os.path.join(dp, f), errors="ignore").read()
+                assert "oracle" not in txt.lower(), f

@@ -1,0 +1,196 @@
+"""HIP path vs the oracle through the C-ABI, phase by phase and end to end.
+
+FP64 bar: 1e-10 relative (per element, relative to the array's max
+magnitude) for every phase; the multi-step trajectory compares at 1e-9.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import miniafqmc_amd as m
+import oracle_lib as orc
+from miniafqmc_amd import _native as nat
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def close(a, b, rtol=RTOL, what=""):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / scale
+    assert err <= rtol, f"{what}: max rel err {err:.3e} > {rtol:.1e}"
+    return err
+
+
+@pytest.fixture(scope="module")
+def c1(require_gpu):
+    s, p = m.preset_system("N16_M64")
+    return s, p
+
+
+def perturbed_walkers(s, nw, seed=3, noise=0.1):
+    r = np.random.default_rng(seed)
+    W = np.zeros((nw, 2, s.nmo, s.naea), complex)
+    W[:, 0] = s.trial_alpha.conj()
+    W[:, 1] = s.trial_beta.conj()
+    W += noise * (r.standard_normal(W.shape) + 1j * r.standard_normal(W.shape))
+    return W
+
+
+def test_halfrotate(c1):
+    s, p = c1
+    say("halfrotate N16_M64")
+    g = m.AfqmcSys(s, 4, "fp64_csr")
+    spT_g = g.spvnT()
+    spT_o = orc.halfrotate(s, cutoff=1e-6)
+    Dg, Do = spT_g.dense(), spT_o.dense()
+    both = (Dg != 0) & (Do != 0)
+    close(Dg[both], Do[both], 1e-12, "SpvnT values")
+    only = (Dg != 0) ^ (Do != 0)
+    assert np.all(np.abs(np.abs(np.where(only, Dg + Do, 0))[only] - 1e-6) < 1e-13)
+    assert abs(spT_g.nnz - spT_o.nnz) <= 2
+    for r in range(s.nchol):
+        c = spT_g.col[spT_g.rowptr[r]:spT_g.rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+    g.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense"])
+def test_mixed_density_matrix(c1, mode):
+    s, p = c1
+    nw = p["nwalk"]
+    say("mdm", mode)
+    g = m.AfqmcSys(s, nw, mode)
+    W = perturbed_walkers(s, nw)
+    Wd = np.zeros((nw, 8), complex)
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_WDATA, Wd)
+    G = g.calculate_mixed_density_matrix()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    close(G, Go, what="G")
+    close(g.W_data[:, 2:4], Wd[:, 2:4], what="overlaps")
+    g.close()
+
+
+def test_phases_fp64_csr(c1):
+    """vbias, fields, vHS, propagate, overlaps, weights, ortho, energy with shared inputs."""
+    s, p = c1
+    nw = p["nwalk"]
+    g = m.AfqmcSys(s, nw, "fp64_csr")
+    spT = orc.halfrotate(s)
+    W = perturbed_walkers(s, nw, seed=5)
+    Wd = np.zeros((nw, 8), complex)
+    Wd[:, 1] = 1.0
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_WDATA, Wd)
+    say("phases: mdm")
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    g.calculate_mixed_density_matrix()
+    say("phases: vbias")
+    g.set(nat.BUF_G, Go)
+    vb = g.get_vbias()
+    vbo = orc.get_vbias(spT, Go)
+    close(vb, vbo, what="vbias")
+    say("phases: fields")
+    x = m.fields(p["seed"], s.nchol, nw, 0)
+    g.set(nat.BUF_VBIAS, vbo)
+    g.fields(x)
+    Xo, ho = orc.fields(vbo, x)
+    close(g.get(nat.BUF_X), Xo, what="X")
+    close(g.get(nat.BUF_HYBRIDW), ho, what="hybridW")
+    say("phases: vHS")
+    g.set(nat.BUF_X, Xo)
+    vhs = g.get_vHS()
+    vhso = orc.get_vhs(s.spvn, Xo)
+    close(vhs, vhso, what="vHS")
+    say("phases: propagate")
+    g.set(nat.BUF_VHS, vhso)
+    g.propagate()
+    Wo = orc.propagate(s, W, vhso)
+    close(g.W, Wo, what="propagated W")
+    say("phases: overlaps + weights")
+    g.set(nat.BUF_W, Wo)
+    g.save_overlaps()
+    g.calculate_overlaps()
+    Wd[:, 5] = Wd[:, 4]
+    Wd[:, 6:8] = Wd[:, 2:4]
+    orc.calculate_overlaps(s, Wo, Wd)
+    close(g.W_data[:, 2:4], Wd[:, 2:4], what="overlaps")
+    g.set(nat.BUF_HYBRIDW, ho)
+    es = g.weights(s.dt, 0.25)
+    eso = orc.weights(s.dt, ho, Wd, 0.25)
+    close(g.W_data, Wd, what="W_data after weights")
+    assert abs(es - eso) <= 1e-10 * abs(eso)
+    say("phases: orthogonalize")
+    g.orthogonalize()
+    Woo = orc.orthogonalize(Wo)
+    close(g.W, Woo, what="orthogonalized W")
+    say("phases: energy")
+    g.set(nat.BUF_WDATA, Wd)
+    g.calculate_mixed_density_matrix()
+    e = g.calculate_energy()
+    G2, _ = orc.calculate_mixed_density_matrix(s, Woo, Wd)
+    eo = orc.calculate_energy(s, Wd, G2)
+    close(g.W_data[:, 0], Wd[:, 0], what="E_L (Vakbl)")
+    assert abs(e - eo) <= 1e-10 * abs(eo)
+    g.close()
+
+
+def run_both(s, p, mode, nsteps=None, nw=None, cutoff=1e-6):
+    nw = nw or p["nwalk"]
+    ns = nsteps or p["nsteps"]
+    nsub, northo = p["nsubsteps"], p["northo"]
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, ns * nsub)
+    g = m.AfqmcSys(s, nw, mode, cutoff=cutoff)
+    g.init_walkers()
+    g.stage_fields(x)
+    g.calculate_mixed_density_matrix()
+    e0 = g.calculate_energy()
+    eav = g.run(ns, nsub, northo, s.dt)
+    res = dict(eav=np.concatenate([[e0], eav]), W=g.W, Wd=g.W_data, eshift=g.eshift())
+    g.close()
+    spT = orc.halfrotate(s, cutoff=cutoff)
+    Lt = orc.dense_Lt(s)
+    emode = 0 if (mode == "fp64_csr" and s.vakbl is not None) else 1
+    eo, Wo, Wdo, eso, tm, info = orc.run(s, nw, ns, nsub, northo, x, energy_mode=emode, spT=spT, Lt=Lt)
+    return res, dict(eav=eo, W=Wo, Wd=Wdo, eshift=eso)
+
+
+def test_run_fp64_csr_matches_oracle(c1):
+    s, p = c1
+    say("run fp64_csr N16_M64 20 steps")
+    g, o = run_both(s, p, "fp64_csr")
+    close(g["eav"], o["eav"], 1e-9, "Eav trajectory")
+    close(g["W"], o["W"], 1e-9, "final walkers")
+    close(g["Wd"][:, 0], o["Wd"][:, 0], 1e-9, "final E_L")
+    close(g["Wd"][:, 1], o["Wd"][:, 1], 1e-9, "final weights")
+    close(g["Wd"][:, 2:4], o["Wd"][:, 2:4], 1e-9, "final overlaps")
+    assert abs(g["eshift"] - o["eshift"]) <= 1e-9 * abs(o["eshift"])
+
+
+def test_run_fp64_dense_matches_oracle(c1):
+    s, p = c1
+    say("run fp64_dense N16_M64 20 steps")
+    g, o = run_both(s, p, "fp64_dense", cutoff=-1.0)
+    close(g["eav"], o["eav"], 1e-9, "Eav trajectory")
+    close(g["W"], o["W"], 1e-9, "final walkers")
+    close(g["Wd"][:, 0], o["Wd"][:, 0], 1e-9, "final E_L")
+
+
+@pytest.mark.parametrize("mode,tol", [("mixed_fp32", 1e-3), ("mixed_bf16", 1e-3)])
+def test_run_mixed_error_vs_fp64_oracle(c1, mode, tol):
+    s, p = c1
+    say("run", mode, "N16_M64 20 steps")
+    g, o = run_both(s, p, mode, cutoff=-1.0)
+    dE = np.abs(g["Wd"][:, 0] - o["Wd"][:, 0])
+    dEav = np.abs(g["eav"] - o["eav"])
+    say(f"{mode}: max |dE_L| = {dE.max():.3e} Ha, max |dEav| = {dEav.max():.3e} Ha")
+    assert dE.max() < tol and dEav.max() < tol

@@ -1,0 +1,278 @@
+"""Pins the oracle's AFQMC functions against closed forms, invariants and
+numpy/scipy restatements on small synthetic systems."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import miniafqmc_amd as m
+import oracle_lib as orc
+
+rng = np.random.default_rng(11)
+
+
+def crand(*shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.fixture(scope="module")
+def small():
+    # 24 orbitals, 6+6 electrons, 40 vectors, Vakbl included
+    return m.make_system(24, 6, 40, density=0.4, sigma_L=0.05, sigma_h=1.0, dt=0.01, seed=5, with_vakbl=True)
+
+
+def walkers(s, nw, noise=0.2):
+    W = np.zeros((nw, 2, s.nmo, s.naea), complex)
+    for w in range(nw):
+        W[w, 0] = s.trial_alpha.conj() + noise * crand(s.nmo, s.naea)
+        W[w, 1] = s.trial_beta.conj() + noise * crand(s.nmo, s.naea)
+    return W
+
+
+# ---- synthetic system --------------------------------------------------------
+def test_synthetic_system_structure(small):
+    s = small
+    M, N, K = s.nmo, s.naea, s.nchol
+    for t in (s.trial_alpha, s.trial_beta):
+        np.testing.assert_allclose(t.conj().T @ t, np.eye(N), atol=1e-13)  # orthonormal orbitals
+    assert not np.allclose(s.trial_alpha, s.trial_beta)
+    np.testing.assert_allclose(s.h1, s.h1.conj().T, atol=1e-15)
+    np.testing.assert_allclose(s.propg1, sla.expm(-0.5 * s.dt * s.h1), rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(s.haj[:N], s.trial_alpha.T @ s.h1, rtol=1e-12, atol=1e-13)
+    np.testing.assert_allclose(s.haj[N:], s.trial_beta.T @ s.h1, rtol=1e-12, atol=1e-13)
+    L = s.spvn.dense().reshape(M, M, K)
+    np.testing.assert_allclose(L, L.conj().transpose(1, 0, 2), atol=1e-15)  # Hermitian vectors
+    dens = s.spvn.nnz / (M * M * K)
+    assert 0.3 < dens < 0.5
+    # CSR sorted by (row, col)
+    for r in range(0, M * M, 37):
+        c = s.spvn.col[s.spvn.rowptr[r]:s.spvn.rowptr[r + 1]]
+        assert np.all(np.diff(c) > 0)
+
+
+def test_fields_are_walker_columns():
+    full = m.fields(3, 10, 8, substep=4)
+    part = m.fields(3, 10, 3, substep=4, w0=2, wtot=8)
+    assert np.array_equal(full[:, 2:5], part)
+    x = m.fields(9, 4000, 64, substep=0)
+    assert abs(x.mean()) < 0.02 and abs(x.std() - 1) < 0.02
+
+
+# ---- mixed density matrix ----------------------------------------------------
+def test_mixed_density_matrix_invariants(small):
+    s = small
+    B = s.trial_alpha.conj() + 0.3 * crand(s.nmo, s.naea)
+    Gc, ov, info = orc.mixed_density_matrix(s.trial_alpha, B)
+    assert info == 0
+    T1 = B.T @ s.trial_alpha
+    np.testing.assert_allclose(Gc, np.linalg.inv(T1) @ B.T, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(ov, np.linalg.det(T1), rtol=1e-11)
+    np.testing.assert_allclose(Gc @ s.trial_alpha, np.eye(s.naea), atol=1e-12)  # Gc conj(Psi) = I
+    Gf, ov2, _ = orc.mixed_density_matrix(s.trial_alpha, B, compact=False)
+    np.testing.assert_allclose(Gf, s.trial_alpha @ Gc, rtol=1e-11, atol=1e-12)  # full = conjA Gc
+    np.testing.assert_allclose(Gf @ Gf, Gf, atol=1e-10)  # idempotent
+    np.testing.assert_allclose(np.trace(Gf), s.naea, rtol=1e-12)
+    ov3, _ = orc.overlap(s.trial_alpha, B)
+    np.testing.assert_allclose(ov3, ov, rtol=1e-13)
+
+
+def test_mixed_density_matrix_at_trial(small):
+    s = small
+    nw = 3
+    W = np.stack([np.stack([s.trial_alpha.conj(), s.trial_beta.conj()])] * nw)
+    Wd = np.zeros((nw, 8), complex)
+    G, info = orc.calculate_mixed_density_matrix(s, W, Wd)
+    np.testing.assert_allclose(Wd[:, 2:4], 1.0, atol=1e-13)
+    G4 = G.reshape(2, s.naea, s.nmo, nw)
+    np.testing.assert_allclose(G4[0, :, :, 1], s.trial_alpha.conj().T, atol=1e-13)  # Gc = Psi^T
+    np.testing.assert_allclose(G4[1, :, :, 2], s.trial_beta.conj().T, atol=1e-13)
+
+
+# ---- vbias / vHS ----------------------------------------------------------------
+def test_halfrotate_brute_force(small):
+    s = small
+    M, N, K = s.nmo, s.naea, s.nchol
+    L = s.spvn.dense()  # [M*M][K]
+    ref = np.zeros((K, 2 * N * M), complex)
+    for sp, t in enumerate((s.trial_alpha, s.trial_beta)):
+        for n in range(K):
+            ref[n, sp * N * M:(sp + 1) * N * M] = (t.T @ L[:, n].reshape(M, M)).ravel()
+    spT = orc.halfrotate(s, cutoff=-1.0)
+    np.testing.assert_allclose(spT.dense(), ref, rtol=1e-12, atol=1e-15)
+    cut = 3e-3
+    spTc = orc.halfrotate(s, cutoff=cut)
+    D = spTc.dense()
+    keep = np.abs(ref) > cut
+    borderline = np.abs(np.abs(ref) - cut) < 1e-12
+    assert np.all((D != 0) == keep | (borderline & (D != 0)))
+    np.testing.assert_allclose(D[keep], ref[keep], rtol=1e-12)
+
+
+def test_vbias_transposed_equals_full_path(small):
+    s = small
+    nw = 4
+    W = walkers(s, nw)
+    Wd = np.zeros((nw, 8), complex)
+    Gc, _ = orc.calculate_mixed_density_matrix(s, W, Wd, compact=True)
+    Gf, _ = orc.calculate_mixed_density_matrix(s, W, Wd.copy(), compact=False)
+    spT = orc.halfrotate(s, cutoff=-1.0)
+    vt = orc.get_vbias(spT, Gc, transposed=True)
+    vn = orc.get_vbias(s.spvn, Gf, transposed=False)
+    np.testing.assert_allclose(vt, vn, rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(vt, spT.dense() @ Gc, rtol=1e-12, atol=1e-14)
+
+
+def test_vhs_brute_force(small):
+    s = small
+    X = crand(s.nchol, 5)
+    np.testing.assert_allclose(orc.get_vhs(s.spvn, X), s.spvn.dense() @ X, rtol=1e-12, atol=1e-14)
+
+
+# ---- fields / weights ------------------------------------------------------------
+def test_fields_closed_form():
+    vb = crand(6, 3)
+    x = rng.standard_normal((6, 3))
+    X, h = orc.fields(vb, x)
+    np.testing.assert_allclose(X, x + 1j * vb, rtol=1e-15)
+    np.testing.assert_allclose(h, (0.5 * vb * vb - 1j * vb * x).sum(0), rtol=1e-13)
+
+
+def test_weights_closed_form():
+    dt, es = 0.01, 0.3
+    Wd = np.zeros((3, 8), complex)
+    Wd[:, 1] = 2.0
+    Wd[:, 5] = [1.0, -2.0, 0.5]
+    # new overlaps / old overlaps: ratios 2 (real), -1 (phase pi), 1j (phase pi/2)
+    Wd[:, 2:4] = [[2.0, 1.0], [-1.0, 1.0], [1j, 1.0]]
+    Wd[:, 6:8] = 1.0
+    hyb = np.array([0.1 + 0.2j, 0.0, -0.3j])
+    es2 = orc.weights(dt, hyb, Wd, es)
+    ratio = np.array([2.0, -1.0, 1j])
+    e4 = -(hyb + np.log(ratio)) / dt
+    np.testing.assert_allclose(Wd[:, 4], e4, rtol=1e-14)
+    scale = np.maximum(0, np.cos(np.angle(ratio)))
+    np.testing.assert_allclose(Wd[:, 1].real, 2 * scale * np.exp(-dt * (0.5 * (e4.real + [1.0, -2.0, 0.5]) - es)),
+                               rtol=1e-14, atol=1e-300)
+    assert Wd[1, 1] == 0 and abs(Wd[2, 1]) < 1e-15  # phase pi and pi/2 kill the weight
+    np.testing.assert_allclose(es2, e4.real.mean(), rtol=1e-14)
+
+
+# ---- propagation ------------------------------------------------------------------
+def test_apply_expM_closed_forms():
+    S = crand(5, 3)
+    np.testing.assert_array_equal(orc.apply_expM(np.zeros((5, 5)), S), S)  # V = 0
+    lam = crand(5) * 0.3
+    out = orc.apply_expM(np.diag(lam), S)
+    poly = sum((1j * lam) ** n / math.factorial(n) for n in range(7))
+    np.testing.assert_allclose(out, poly[:, None] * S, rtol=1e-13)
+    V = 0.2 * crand(5, 5)
+    ref = sum(np.linalg.matrix_power(1j * V, n) / math.factorial(n) for n in range(7)) @ S
+    np.testing.assert_allclose(orc.apply_expM(V, S), ref, rtol=1e-12)
+    # truncation bound against the exact exponential: ||V||^7/7! scale
+    err = np.linalg.norm(orc.apply_expM(V, S) - sla.expm(1j * V) @ S)
+    assert err <= 2 * np.linalg.norm(V, 2) ** 7 / 5040 * np.linalg.norm(S) * np.e
+
+
+def test_propagate_brute_force(small):
+    s = small
+    nw = 2
+    W = walkers(s, nw)
+    vHS = 0.05 * crand(s.nmo * s.nmo, nw)
+    out = orc.propagate(s, W, vHS)
+    for w in range(nw):
+        V = vHS[:, w].reshape(s.nmo, s.nmo)
+        E = sum(np.linalg.matrix_power(1j * V, n) / math.factorial(n) for n in range(7))
+        for sp in range(2):
+            np.testing.assert_allclose(out[w, sp], s.propg1 @ E @ s.propg1 @ W[w, sp], rtol=1e-11, atol=1e-13)
+
+
+def test_orthogonalize_invariants(small):
+    s = small
+    nw = 3
+    W = walkers(s, nw, noise=1.0)
+    Wd0 = np.zeros((nw, 8), complex)
+    G0, _ = orc.calculate_mixed_density_matrix(s, W, Wd0)
+    Wo = orc.orthogonalize(W)
+    for w in range(nw):
+        for sp in range(2):
+            np.testing.assert_allclose(Wo[w, sp].T @ Wo[w, sp].conj(), np.eye(s.naea), atol=1e-13)
+            # W = Wo R with R upper triangular (LQ of the transpose)
+            R = Wo[w, sp].conj().T @ W[w, sp]
+            np.testing.assert_allclose(np.tril(R, -1), 0, atol=1e-12)
+    Wd1 = np.zeros((nw, 8), complex)
+    G1, _ = orc.calculate_mixed_density_matrix(s, Wo, Wd1)
+    np.testing.assert_allclose(G1, G0, rtol=1e-10, atol=1e-12)  # G is gauge invariant
+    for w in range(nw):
+        for sp in range(2):
+            R = Wo[w, sp].conj().T @ W[w, sp]
+            np.testing.assert_allclose(Wd0[w, 2 + sp] / Wd1[w, 2 + sp], np.prod(np.diag(R)), rtol=1e-10)
+
+
+# ---- energy ------------------------------------------------------------------------
+def test_energy_vakbl_equals_cholesky_form(small):
+    s = small
+    nw = 4
+    W = walkers(s, nw)
+    Wd = np.zeros((nw, 8), complex)
+    Wd[:, 1] = [1.0, 0.5, 2.0, 0.25]
+    G, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    Wa, Wb = Wd.copy(), Wd.copy()
+    ea = orc.calculate_energy(s, Wa, G)
+    eb = orc.energy_cholesky(s, Wb, G)
+    np.testing.assert_allclose(Wa[:, 0], Wb[:, 0], rtol=1e-11)
+    np.testing.assert_allclose(ea, eb, rtol=1e-11)
+    np.testing.assert_allclose(ea, (Wa[:, 0].real * Wa[:, 1].real).sum() / Wa[:, 1].real.sum(), rtol=1e-14)
+    # brute force: E1 = sum Gc haj, E2 = 1/2 Gc^T Vakbl Gc
+    V = s.vakbl.dense()
+    for w in range(nw):
+        g = G[:, w]
+        np.testing.assert_allclose(Wa[w, 0], 0.5 * g @ V @ g + g @ s.haj.ravel(), rtol=1e-11)
+
+
+def test_energy_at_trial_is_real(small):
+    # Hermitian vectors and h: the trial's own energy is real
+    s = small
+    W = np.stack([np.stack([s.trial_alpha.conj(), s.trial_beta.conj()])])
+    Wd = np.zeros((1, 8), complex)
+    Wd[0, 1] = 1
+    G, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    orc.calculate_energy(s, Wd, G)
+    assert abs(Wd[0, 0].imag) < 1e-10 * abs(Wd[0, 0])
+
+
+# ---- driver loop ---------------------------------------------------------------------
+def test_run_composes_the_pieces(small):
+    """Two substeps of orc_run equal the pieces called in the driver's order."""
+    s = small
+    nw, nsub = 3, 2
+    x = m.fields_block(17, s.nchol, nw, 0, nsub)
+    eav, Wr, Wdr, esr, tm, info = orc.run(s, nw, 1, nsub, 1, x)
+    assert info == 0
+    spT = orc.halfrotate(s)
+    W = np.stack([np.stack([s.trial_alpha.conj(), s.trial_beta.conj()])] * nw)
+    Wd = np.zeros((nw, 8), complex)
+    Wd[:, 1] = 1
+    G, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    e0 = orc.calculate_energy(s, Wd, G)
+    np.testing.assert_allclose(eav[0], e0, rtol=1e-13)
+    es = 0.0
+    for t in range(nsub):
+        G, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+        vb = orc.get_vbias(spT, G)
+        X, h = orc.fields(vb, x[t])
+        vhs = orc.get_vhs(s.spvn, X)
+        W = orc.propagate(s, W, vhs)
+        Wd[:, 5] = Wd[:, 4]
+        Wd[:, 6:8] = Wd[:, 2:4]
+        orc.calculate_overlaps(s, W, Wd)
+        es = orc.weights(s.dt, h, Wd, es)
+        if t > 0 and t % 1 == 0:
+            W = orc.orthogonalize(W)
+            orc.calculate_overlaps(s, W, Wd)
+    G, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    e1 = orc.calculate_energy(s, Wd, G)
+    np.testing.assert_allclose(Wr, W, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(Wdr, Wd, rtol=1e-13, atol=1e-15)
+    np.testing.assert_allclose(eav[1], e1, rtol=1e-13)
+    np.testing.assert_allclose(esr, es, rtol=1e-13)
