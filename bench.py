@@ -1,0 +1,185 @@
+"""Walker-step throughput on one MI355X.
+
+Workload (default): 512 orbitals, 128+128 electrons, 2048 dense Cholesky
+vectors, 1024 walkers, dt = 0.005, synthetic Hamiltonian and auxiliary
+fields (the reference ships no input file).  One step = one substep of all
+walkers through mixed density matrix, vbias, fields, vHS, propagation,
+overlaps and weights; the orthogonalisation runs every `--northo` substeps
+and the local energy once at the end of the timed block, as the
+reference's driver loop places them (miniafqmc_base.cpp:248-350).
+
+Prints one JSON line: walker-steps/s of the chosen precision mode (default
+split-BF16 contractions), the FP64 GPU path on the same workload and the
+per-walker |dE_L| between the two, a roofline entry for the dominant kernel
+(HIP events on the context's stream during the timed block), and the
+oracle's CPU walker-steps/s on a walker sample.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks, TFLOP/s
+    "fp64": 78.6,
+    "fp32": 157.3,
+    "bf16": 2500.0,
+}
+MODE_PEAK = {"fp64_csr": "fp64", "fp64_dense": "fp64", "mixed_fp32": "fp32", "mixed_bf16": "bf16"}
+MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128", "mixed_bf16": "bf16x3+c128"}
+
+
+def kernel_flops(name, M, N, K, nw):
+    """Algorithmic real flops per launch (8 per complex multiply-add)."""
+    c = {
+        "vhs_gemm": nw * M * M * K,
+        "prop_taylor": 2 * nw * M * M * N,
+        "prop_pw": 2 * nw * M * M * N,
+        "prop_ps": 2 * nw * M * M * N,
+        "vbias_gemm": nw * K * N * M,       # one spin block per launch
+        "energy_v_gemm": nw * K * N * M,
+        "ovlp_gemm": 2 * nw * N * N * M,
+        "gc_gemm": 2 * nw * N * N * M,
+        "gj_inverse": 2 * nw * N * N * N,
+    }.get(name)
+    return None if c is None else 8.0 * c
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="N128_M512")
+    ap.add_argument("--mode", default="mixed_bf16")
+    ap.add_argument("--northo", type=int, default=None)
+    ap.add_argument("--walkers", type=int, default=None)
+    ap.add_argument("--fp64", type=int, default=1, help="also run the FP64 GPU path (0 to skip)")
+    ap.add_argument("--cpu", type=int, default=1, help="time the oracle on a walker sample (0 to skip)")
+    ap.add_argument("--cpu-walkers", type=int, default=1)
+    ap.add_argument("--no-energy", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; the walker-sharded multi-GPU "
+              "path is a design note, not built (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import miniafqmc_amd as m
+
+    t0 = time.time()
+    s, p = m.preset_system(args.workload)
+    nw = args.walkers or p["nwalk"]
+    northo = args.northo or p["northo"]
+    M, N, K = s.nmo, s.naea, s.nchol
+    t_gen = time.time() - t0
+    nsub = args.warmup + args.steps
+    x = m.fields_block(p["seed"], K, nw, 0, nsub)
+    print(f"[bench] system M={M} N={N} K={K} nnz={s.spvn.nnz} walkers={nw} generated in {t_gen:.1f}s", flush=True)
+
+    def run_mode(mode, profile):
+        g = m.AfqmcSys(s, nw, mode)
+        g.init_walkers()
+        g.stage_fields(x)
+        g.calculate_mixed_density_matrix()
+        if not args.no_energy:
+            g.calculate_energy()
+        for t in range(args.warmup):
+            g.substep(t, t > 0 and t % northo == 0, s.dt)
+        g.sync()
+        if profile:
+            g.profile(True)
+        ta = time.perf_counter()
+        eav = g.run(1, args.steps, northo, s.dt, first=args.warmup, energy=not args.no_energy)
+        g.sync()
+        tb = time.perf_counter()
+        stats = g.kernel_stats() if profile else {}
+        if profile:
+            g.profile(False)
+        Wd = g.W_data
+        g.close()
+        return tb - ta, eav, Wd, stats
+
+    T, eav, Wd, stats = run_mode(args.mode, True)
+    value = nw * args.steps / T
+    print(f"[bench] {args.mode}: {T:.3f}s for {args.steps} substeps -> {value:.1f} walker-steps/s", flush=True)
+    for k_, (n_, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        print(f"[bench]   {k_:20s} {n_:6d} launches {ms:10.2f} ms total {ms / max(n_, 1):9.3f} ms avg", flush=True)
+
+    # roofline of the dominant kernel with a known algorithmic count
+    roof = None
+    timed = [(k_, v) for k_, v in stats.items() if kernel_flops(k_, M, N, K, nw) is not None]
+    if timed:
+        name, (nl, ms) = max(timed, key=lambda kv: kv[1][1])
+        avg_s = ms / nl / 1e3
+        flops = kernel_flops(name, M, N, K, nw)
+        ach = flops / avg_s / 1e12
+        pk = PEAKS[MODE_PEAK[args.mode] if name not in ("ovlp_gemm", "gc_gemm", "gj_inverse") else "fp64"]
+        roof = {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": pk, "unit": "TFLOP/s",
+                "frac": round(ach / pk, 4), "traffic": None, "avg_ms": round(ms / nl, 4),
+                "flops_per_launch": flops}
+
+    extra = {}
+    if args.fp64 and args.mode != "fp64_dense":
+        T64, eav64, Wd64, _ = run_mode("fp64_dense", False)
+        v64 = nw * args.steps / T64
+        dE = np.abs(Wd[:, 0] - Wd64[:, 0])
+        extra["fp64_gpu"] = {"value": round(v64, 2), "ms_per_step": round(1e3 * T64 / args.steps, 3)}
+        extra["speedup_vs_fp64"] = round(value / v64, 3)
+        if not args.no_energy:
+            extra["max_abs_dE_L_Ha"] = float(dE.max())
+            extra["mean_abs_dE_L_Ha"] = float(dE.mean())
+            extra["dEav_Ha"] = float(abs(eav[-1] - eav64[-1]))
+            extra["E_L_mean_Ha"] = float(Wd64[:, 0].real.mean())
+        print(f"[bench] fp64_dense: {v64:.1f} walker-steps/s; mixed/fp64 = {value / v64:.2f}x; "
+              f"max|dE_L| = {extra.get('max_abs_dE_L_Ha')}", flush=True)
+
+    cpu = None
+    if args.cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as orc
+        cw = args.cpu_walkers
+        tr = time.time()
+        spT = orc.halfrotate(s, cutoff=1e-6)
+        print(f"[bench] oracle half rotation {time.time() - tr:.1f}s (setup, untimed)", flush=True)
+        xs = x[:1, :, :cw].copy()
+        eo, Wo, Wdo, eso, tm, info = orc.run(s, cw, 1, 1, 1 << 30, xs, energy_mode=2, spT=spT)
+        tcpu = tm[0]
+        cpu = {"value": round(cw / tcpu, 4), "unit": "walker-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{cw} walker(s) x 1 substep of the same workload (no energy), "
+                         f"{tcpu:.1f}s on 1 host core; oracle = FP64 C++ restatement, CSR SpvnT/Spvn"}
+        print(f"[bench] cpu oracle: {tcpu:.2f}s per {cw} walker-substep(s) -> {cw / tcpu:.3f} walker-steps/s",
+              flush=True)
+
+    line = {
+        "metric": "walker-steps/s (M=512, N=128+128, nchol=2048, 1024 walkers)",
+        "value": round(value, 2),
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * T / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": MODE_DTYPE[args.mode],
+        "data": "synthetic (seeded Hamiltonian and auxiliary fields)",
+        "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
+                   "mode": args.mode, "northo": northo,
+                   "energy": "none" if args.no_energy else f"once per {args.steps} substeps",
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -760,8 +760,7 @@ int orc_weights(int nw, double dt, const double* hybridW, double* Wd, double* es
 
 /* base::halfrotate_cholesky (rotate.hpp:65-169).  The reference counts in a
  * first pass and fills in a second with identical arithmetic; one pass that
- * keeps the kept entries gives the same matrix.  The rotation of each vector
- * runs over the stored nonzeros of that vector (same sum as the dense gemm). */
+ * keeps the kept entries gives the same matrix. */
 long orc_halfrotate_cholesky(int M, int N, int K, const double* ta_, const double* tb_, const int* rp,
                              const int* col, const double* val, double cutoff, int* out_rp, int* out_col,
                              double* out_val)
@@ -775,19 +774,33 @@ long orc_halfrotate_cholesky(int M, int N, int K, const double* ta_, const doubl
   std::vector<std::vector<cd>> rv(K);
 #pragma omp parallel
   {
-    std::vector<cd> C((size_t)N * M);
+    std::vector<cd> C((size_t)N * M), An((size_t)M * M);
+    std::vector<double> Cr((size_t)M), Ci((size_t)M);
 #pragma omp for schedule(dynamic)
     for (int i = 0; i < K; i++) {
       int nterms = trp[i + 1] - trp[i];
       if (nterms == 0) continue;
+      // extract Cholesky vector i as a dense M x M matrix (rotate.hpp:103-111)
+      std::fill(An.begin(), An.end(), cd(0.0));
+      for (int p = trp[i]; p < trp[i + 1]; p++) An[tci[p]] = tv[p];
       for (int sp = 0; sp < 2; sp++) {
         const cd* al = sp == 0 ? ta : tb;
-        std::fill(C.begin(), C.end(), cd(0.0));
-        // C = T(alpha) * An :  C[a][k] = sum_i alpha[i][a] An[i][k]
-        for (int p = trp[i]; p < trp[i + 1]; p++) {
-          int ii = tci[p] / M, kk = tci[p] % M;
-          cd v = tv[p];
-          for (int a = 0; a < N; a++) C[(size_t)a * M + kk] += cmul(al[(size_t)ii * N + a], v);
+        // C = T(alpha) * An :  C[a][k] = sum_i alpha[i][a] An[i][k]  (rotate.hpp:116,122)
+        for (int a = 0; a < N; a++) {
+          std::fill(Cr.begin(), Cr.end(), 0.0);
+          std::fill(Ci.begin(), Ci.end(), 0.0);
+          for (int ii = 0; ii < M; ii++) {
+            const cd c = al[(size_t)ii * N + a];
+            if (c == cd(0.0)) continue;
+            const double cr = c.real(), ci = c.imag();
+            const cd* row = An.data() + (size_t)ii * M;
+            for (int k = 0; k < M; k++) {
+              const double xr = row[k].real(), xi = row[k].imag();
+              Cr[k] += cr * xr - ci * xi;
+              Ci[k] += cr * xi + ci * xr;
+            }
+          }
+          for (int k = 0; k < M; k++) C[(size_t)a * M + k] = cd(Cr[k], Ci[k]);
         }
         for (int a = 0; a < N; a++)
           for (int k = 0; k < M; k++)
