@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--cpu", type=int, default=1, help="time the oracle on a walker sample (0 to skip)")
     ap.add_argument("--cpu-walkers", type=int, default=1)
     ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -85,8 +86,12 @@ def main():
 
     def run_mode(mode, profile):
         g = m.AfqmcSys(s, nw, mode)
-        g.init_walkers()
         g.stage_fields(x)
+        if args.tune:
+            tt = time.perf_counter()
+            g.tune(s.dt)
+            print(f"[bench] {mode}: tile autotune {time.perf_counter() - tt:.1f}s (untimed)", flush=True)
+        g.init_walkers()
         g.calculate_mixed_density_matrix()
         if not args.no_energy:
             g.calculate_energy()

@@ -76,6 +76,15 @@ int afqmc_ctx_destroy(afqmc_ctx* ctx);
 const char* afqmc_ctx_error(const afqmc_ctx* ctx);
 int afqmc_sync(afqmc_ctx* ctx);
 
+/* Options (set before afqmc_upload_system). */
+typedef enum {
+  /* 1 (default): keep the same-spin exchange block of Vakbl dense in device
+   * memory, [2][NM][NM] in the contraction precision, when it fits, so the
+   * two-body energy is one GEMM per spin; 0: always use the Cholesky form. */
+  AFQMC_OPT_DENSE_EXCHANGE = 0
+} afqmc_option;
+int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
+
 /* Replaces the data hand-off of afqmc::Initialize
  * (src/Matrix/initialize_serial.hpp:35-228) followed by
  * base::halfrotate_cholesky (src/AFQMC/rotate.hpp:65-169, called from
@@ -138,7 +147,8 @@ int afqmc_energy(afqmc_ctx* ctx, double* eav_out);
  * (src/Numerics/ma_operations.hpp:94-107 -> BLAS::gemm,
  * src/Numerics/OhmmsBlas.h:270-278): C[m][n] = alpha*op(A)*op(B) + beta*C,
  * op in {'N','T'}, host arrays.  precision: 0 FP64, 1 FP32, 2 split-BF16
- * (the arithmetic of the modes above). */
+ * (the arithmetic of the modes above); bits 8..15 of `precision` may force
+ * one tile candidate (1-based) instead of the size heuristic. */
 int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int k, const double* alpha, const double* A,
                 int lda, const double* B, int ldb, const double* beta, double* C, int ldc);
 
@@ -160,6 +170,11 @@ int afqmc_set_eshift(afqmc_ctx* ctx, double eshift);
  * each block into eav_out[nsteps] (may be NULL to skip the energy). */
 int afqmc_run(afqmc_ctx* ctx, long first_substep, int nsteps, int nsubsteps, int northo, double dt,
               double* eav_out);
+/* Tile autotuning: runs one substep (fields of staged substep 0) and one
+ * energy evaluation, timing every tile candidate of each GEMM call site and
+ * keeping the fastest for this context.  The walker set, W_data and Eshift
+ * are undefined afterwards: call afqmc_init_walkers (or afqmc_copy) next. */
+int afqmc_tune(afqmc_ctx* ctx, double dt);
 /* Capture one substep (with and without orthogonalisation) into HIP graphs
  * and replay them from then on in afqmc_substep/afqmc_run (enable != 0). */
 int afqmc_use_graphs(afqmc_ctx* ctx, int enable);

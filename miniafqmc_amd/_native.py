@@ -28,12 +28,12 @@ BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW = range(7)
 
 # every symbol include/afqmc_hip.h declares
 EXPORTED = [
-    "afqmc_ctx_create", "afqmc_ctx_destroy", "afqmc_ctx_error", "afqmc_sync",
+    "afqmc_ctx_create", "afqmc_ctx_destroy", "afqmc_ctx_error", "afqmc_sync", "afqmc_set_option",
     "afqmc_upload_system", "afqmc_get_spvnT", "afqmc_init_walkers", "afqmc_copy",
     "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_fields", "afqmc_vhs",
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
     "afqmc_weights", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
-    "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run",
+    "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
     "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats", "afqmc_zgemm",
     "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
 ]
@@ -93,6 +93,7 @@ def _load():
         "afqmc_ctx_destroy": (I, [P]),
         "afqmc_ctx_error": (C.c_char_p, [P]),
         "afqmc_sync": (I, [P]),
+        "afqmc_set_option": (I, [P, I, C.c_double]),
         "afqmc_upload_system": (I, [P, D, D, D, D, C.POINTER(CsrView), C.POINTER(CsrView), C.c_double, C.c_double]),
         "afqmc_get_spvnT": (I, [P, C.POINTER(C.c_longlong), C.POINTER(C.c_int), C.POINTER(C.c_int), D]),
         "afqmc_init_walkers": (I, [P]),
@@ -112,6 +113,7 @@ def _load():
         "afqmc_get_eshift": (I, [P, D]),
         "afqmc_set_eshift": (I, [P, C.c_double]),
         "afqmc_run": (I, [P, L, I, I, I, C.c_double, D]),
+        "afqmc_tune": (I, [P, C.c_double]),
         "afqmc_use_graphs": (I, [P, I]),
         "afqmc_profile": (I, [P, I]),
         "afqmc_kernel_stats": (I, [P, I, P, C.POINTER(C.c_longlong), D, C.POINTER(C.c_int)]),

@@ -23,13 +23,14 @@ def _dp(a):
 
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
-                 use_vakbl: bool = True):
+                 use_vakbl: bool = True, dense_exchange: bool = True):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
         h = C.c_void_p()
         nat.check(nat.lib.afqmc_ctx_create(self.M, self.N, self.K, nwalk, nat.MODES[mode], C.byref(h)))
         self.ctx = h
+        self._check(nat.lib.afqmc_set_option(self.ctx, 0, 1.0 if dense_exchange else 0.0))
         ta = np.ascontiguousarray(system.trial_alpha)
         tb = np.ascontiguousarray(system.trial_beta)
         sp = system.spvn.view()
@@ -171,6 +172,10 @@ class AfqmcSys:
         self._check(nat.lib.afqmc_run(self.ctx, first, nsteps, nsubsteps, northo, dt,
                                       _dp(eav) if energy else None))
         return eav[:nsteps]
+
+    def tune(self, dt):
+        """Time every GEMM tile candidate once per call site; re-initialise walkers afterwards."""
+        self._check(nat.lib.afqmc_tune(self.ctx, dt))
 
     def profile(self, on=True):
         self._check(nat.lib.afqmc_profile(self.ctx, int(on)))

@@ -70,7 +70,9 @@ struct afqmc_ctx {
   // walker state and per-substep buffers
   zd* W = nullptr;
   zd* Wd = nullptr;
-  zd* T1 = nullptr;       // [nw][2][N][N]
+  zd* T1 = nullptr;       // [nw][2][N][N] overlap matrices, then their inverses
+  zd* dcache = nullptr;   // [nw][2] determinants of the last inversion
+  bool t1inv_valid = false;  // T1 holds the inverses for the current walkers
   void* G = nullptr;      // CSR: zd [2NM][nw]; FP64_DENSE: zd [nw][2NM]; mixed: zf [nw][2NM]
   zd* vbias = nullptr;    // CSR: [K][nw]; dense: [nw][K]
   void* X = nullptr;      // CSR: zd [K][nw]; dense: [nw][K] (zd / zf)
@@ -93,6 +95,10 @@ struct afqmc_ctx {
   zd* e_v = nullptr;      // [nw][K] direct-term contractions
   void* Ybuf = nullptr;
   long ybuf_elems = 0;
+  // same-spin exchange block of Vakbl, dense [2][NM][NM] (scaled by dt), in
+  // the contraction precision; null when it does not fit in device memory
+  void* Vx = nullptr;
+  bool allow_vx = true;
   zd* Gtmp = nullptr;     // CSR mode Cholesky energy: G transposed to [nw][2NM]
   zd* Gcloc = nullptr;    // CSR mode Vakbl energy
 
@@ -101,6 +107,11 @@ struct afqmc_ctx {
   std::vector<EvPair> pending;
   std::vector<hipEvent_t> evpool;
   std::map<std::string, KStat> stats;
+
+  // GEMM tile autotuning: call-site key -> candidate index
+  std::map<std::string, int> tune;
+  bool tuning = false;
+  int force_cfg = 0;  // 1-based tile candidate forced by afqmc_zgemm (tests)
 
   // graphs
   bool use_graphs = false;
@@ -208,31 +219,93 @@ void collect_events(afqmc_ctx* c)
 inline int nblk(long n, int t) { return (int)std::min<long>((n + t - 1) / t, 65535L * 8); }
 
 // ---------------------------------------------------------------------------
-// GEMM dispatch.  Small problems use a 1-wave 32x16 tile, large ones a 4-wave
-// 128x64 tile (complex columns).
+// GEMM dispatch.  Each call site (name and shape) runs one of a few tile
+// shapes; afqmc_tune() times all candidates once per call site and keeps the
+// fastest, otherwise a size heuristic picks.
+struct TileCfg {
+  int BM, BNC, WM, WN;
+};
+// candidates per precision (BKC is 8 complex for FP64/FP32, 16 for BF16)
+constexpr TileCfg kCfgF64[] = {{32, 16, 1, 1}, {64, 32, 2, 2}, {128, 64, 2, 2}};
+constexpr TileCfg kCfgF32[] = {{32, 16, 1, 1}, {128, 64, 2, 2}, {128, 128, 2, 2}};
+constexpr TileCfg kCfgB16[] = {{32, 16, 1, 1}, {128, 64, 2, 2}, {128, 128, 2, 2}, {256, 64, 4, 2}, {64, 64, 2, 2}};
+
+template <int CP> constexpr int ncfg() { return CP == CP_F64 ? 3 : (CP == CP_F32 ? 3 : 5); }
+template <int CP> constexpr TileCfg cfg_at(int i) { return CP == CP_F64 ? kCfgF64[i] : (CP == CP_F32 ? kCfgF32[i] : kCfgB16[i]); }
+
+template <int CP, class AT, class BT, class OT, bool AK, bool BK, int CI>
+int zgemm_cfg(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
+{
+  constexpr TileCfg t = cfg_at<CP>(CI);
+  constexpr int BKC = (CP == CP_BF16X3) ? 16 : (t.WM * t.WN == 1 ? 16 : 8);
+  a.MT = (a.m + t.BM - 1) / t.BM;
+  a.NT = (a.n + t.BNC - 1) / t.BNC;
+  const long tiles = (long)a.MT * a.NT * batch;
+  if (tiles > 2147483647L) return fail(c, AFQMC_ERR_SHAPE, "GEMM grid too large");
+  return launch(c, name, [&]() {
+    hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN>), dim3((unsigned)tiles),
+                       dim3(64 * t.WM * t.WN), 0, c->stream, a);
+  });
+}
+
+template <int CP, class AT, class BT, class OT, bool AK, bool BK>
+int zgemm_run(afqmc_ctx* c, const char* name, const ZGemmArgs& a, int batch, int ci)
+{
+  switch (ci) {
+  case 0: return zgemm_cfg<CP, AT, BT, OT, AK, BK, 0>(c, name, a, batch);
+  case 1: return zgemm_cfg<CP, AT, BT, OT, AK, BK, 1>(c, name, a, batch);
+  case 2: return zgemm_cfg<CP, AT, BT, OT, AK, BK, 2>(c, name, a, batch);
+  case 3: if constexpr (ncfg<CP>() > 3) return zgemm_cfg<CP, AT, BT, OT, AK, BK, (ncfg<CP>() > 3 ? 3 : 0)>(c, name, a, batch);
+          break;
+  case 4: if constexpr (ncfg<CP>() > 4) return zgemm_cfg<CP, AT, BT, OT, AK, BK, (ncfg<CP>() > 4 ? 4 : 0)>(c, name, a, batch);
+          break;
+  }
+  return zgemm_cfg<CP, AT, BT, OT, AK, BK, 0>(c, name, a, batch);
+}
+
+template <int CP> int default_cfg(const ZGemmArgs& a)
+{
+  if (a.m >= 96 && a.n >= 48) return CP == CP_F64 ? 2 : 1;  // 128 x 64 tiles
+  return 0;                                                 // one-wave 32 x 16 tiles
+}
+
 template <int CP, class AT, class BT, class OT, bool AK, bool BK>
 int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
 {
   if (a.m <= 0 || a.n <= 0 || batch <= 0) return AFQMC_OK;
   if (a.bdiv <= 0) a.bdiv = 1;
-  const bool large = (a.m >= 96 && a.n >= 48);
-  constexpr int SK = (CP == CP_BF16X3) ? 16 : 8;
-  if (large) {
-    constexpr int BM = 128, BNC = 64, WM = 2, WN = 2;
-    dim3 grid((a.n + BNC - 1) / BNC, (a.m + BM - 1) / BM, batch);
-    return launch(c, name, [&]() {
-      hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, BM, BNC, SK, WM, WN>), grid, dim3(64 * WM * WN), 0,
-                         c->stream, a);
-    });
-  } else {
-    constexpr int BM = 32, BNC = 16, WM = 1, WN = 1;
-    constexpr int BKC = (CP == CP_BF16X3) ? 16 : 16;
-    dim3 grid((a.n + BNC - 1) / BNC, (a.m + BM - 1) / BM, batch);
-    return launch(c, name, [&]() {
-      hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, BM, BNC, BKC, WM, WN>), grid, dim3(64 * WM * WN), 0,
-                         c->stream, a);
-    });
+  char keyb[160];
+  std::snprintf(keyb, sizeof(keyb), "%s/%d/%d/%d/%d/%d/%d%d", name, CP, a.m, a.n, a.k, batch, (int)AK, (int)BK);
+  const std::string key(keyb);
+  auto it = c->tune.find(key);
+  if (it != c->tune.end() && !c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, it->second);
+  if (c->force_cfg > 0 && c->force_cfg <= ncfg<CP>())
+    return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, c->force_cfg - 1);
+  if (!c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, default_cfg<CP>(a));
+  // tuning: time each candidate (results of this call are discarded)
+  const bool prof = c->prof;
+  c->prof = false;
+  hipEvent_t e0 = get_event(c), e1 = get_event(c);
+  int best = default_cfg<CP>(a);
+  float best_ms = 1e30f;
+  for (int ci = 0; ci < ncfg<CP>(); ci++) {
+    RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));  // warm
+    (void)hipEventRecord(e0, c->stream);
+    for (int rep = 0; rep < 3; rep++) RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = ci;
+    }
   }
+  c->evpool.push_back(e0);
+  c->evpool.push_back(e1);
+  c->prof = prof;
+  c->tune[key] = best;
+  return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, best);
 }
 
 ZGemmArgs gargs()
@@ -250,25 +323,26 @@ int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
   const int N = c->N;
   const long batch = 2L * c->nw;
   const long ms = (long)N * N;
+  zd* dc = c->dcache;
   if (N <= 16)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<16, 4>), dim3(batch), dim3(64), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags);
+                         c->flags, dc);
     });
   if (N <= 32)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<32, 8>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags);
+      hipLaunchKernelGGL((gj_inverse_kernel<32, 4>), dim3(batch), dim3(128), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+                         c->flags, dc);
     });
   if (N <= 64)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<64, 8>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags);
+      hipLaunchKernelGGL((gj_inverse_kernel<64, 4>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+                         c->flags, dc);
     });
   if (N <= 128)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<128, 8>), dim3(batch), dim3(1024), 0, c->stream, T, Tinv, det, dq, dr, N,
-                         ms, c->flags);
+      hipLaunchKernelGGL((gj_inverse_kernel<128, 4>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N,
+                         ms, c->flags, dc);
     });
   return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 128 electrons per spin is not supported by the batched inverse");
 }
@@ -304,8 +378,17 @@ int overlap_matrices(afqmc_ctx* c)
 int do_mdm(afqmc_ctx* c)
 {
   const int M = c->M, N = c->N, nw = c->nw;
-  RET(overlap_matrices(c));
-  RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
+  if (c->t1inv_valid) {
+    // the overlap step already inverted W^T conj(Psi) for these walkers
+    RET(launch(c, "det_restore", [&]() {
+      hipLaunchKernelGGL(det_restore_kernel, dim3((2 * nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->dcache,
+                         nw);
+    }));
+  } else {
+    RET(overlap_matrices(c));
+    RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
+    c->t1inv_valid = true;
+  }
   // Gc(a,k) = sum_b T1inv(a,b) W(k,b)
   ZGemmArgs a = gargs();
   a.A = c->T1;
@@ -339,10 +422,14 @@ int do_mdm(afqmc_ctx* c)
   return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
 }
 
+// afqmc_sys::calculate_overlaps; the inverse is kept for the next density
+// matrix of the same walkers.
 int do_overlaps(afqmc_ctx* c)
 {
   RET(overlap_matrices(c));
-  return gj_inverse(c, c->T1, nullptr, c->Wd + 2, 8, 1);
+  RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
+  c->t1inv_valid = true;
+  return AFQMC_OK;
 }
 
 // vbias(w, n) = sum_{s,j} G(w, s, j) Lt(s, n, j) in dense modes.
@@ -505,6 +592,7 @@ int propagate_chain(afqmc_ctx* c, const PT* P, const VT* V)
 
 int do_propagate(afqmc_ctx* c)
 {
+  c->t1inv_valid = false;
   if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHSw);
   if (c->mode == AFQMC_FP64_DENSE)
     return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHS);
@@ -530,10 +618,15 @@ int do_weights(afqmc_ctx* c, const double* dtp, double dt)
   });
 }
 
+constexpr int kLqNB = 8;
+size_t lq_lds_bytes(int M) { return sizeof(zd) * ((size_t)kLqNB * M + 128 * kLqNB + kLqNB * kLqNB + 128 + 4 * kLqNB * kLqNB); }
+
 int do_ortho(afqmc_ctx* c)
 {
+  c->t1inv_valid = false;
+  const size_t lds = lq_lds_bytes(c->M);
   return launch(c, "lq_orthogonalize", [&]() {
-    hipLaunchKernelGGL(lq_kernel, dim3(2 * c->nw), dim3(256), c->N * sizeof(zd), c->stream, c->W, c->M, c->N);
+    hipLaunchKernelGGL(lq_blocked_kernel<kLqNB>, dim3(2 * c->nw), dim3(256), lds, c->stream, c->W, c->M, c->N);
   });
 }
 
@@ -555,6 +648,33 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
     hipLaunchKernelGGL((sumsq_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, c->e_v, 1L, (long)K, c->e_direct,
                        inv_dt);
   }));
+  if (c->Vx) {
+    // exchange from the dense same-spin block of Vakbl held in HBM:
+    // Z(w, p) = sum_q G_s(w, q) Vx_s(p, q);  e_x(w, s) = sum_p G_s(w, p) Z(w, p)
+    using ZT = typename ztype<YT>::type;
+    for (int sp = 0; sp < 2; sp++) {
+      ZGemmArgs a = gargs();
+      a.A = (const GT*)Gp + 2 * (sp * NM);
+      a.sam = 2 * NM;
+      a.sak = 1;
+      a.B = (const YT*)c->Vx + 2 * (sp * NM * NM);
+      a.sbn = NM;
+      a.sbk = 1;
+      a.C = c->Ybuf;
+      a.scm = NM;
+      a.scn = 1;
+      a.m = nw;
+      a.n = (int)NM;
+      a.k = (int)NM;
+      a.ar = inv_dt;
+      RET((zgemm<CP, GT, YT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+      RET(launch(c, "energy_exch_dot", [&]() {
+        hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream, (int)NM,
+                           (const typename ztype<GT>::type*)Gp + sp * NM, 1L, 2 * NM, (const ZT*)c->Ybuf, 1L, NM,
+                           c->e_exch_part + sp, 2L, 1.0, 0);
+      }));
+    }
+  } else {
   // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba, for all
   // walker-spin pairs at once over chunks of nq vectors
   const long per_item = (long)N * N;
@@ -589,6 +709,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
                          2 * nw);
     }));
   }
+  }
   RET(launch(c, "energy_pairsum", [&]() {
     hipLaunchKernelGGL(pairsum_kernel, dim3((nw + 255) / 256), dim3(256), 0, c->stream, c->e_exch_part, c->e_exch, nw);
   }));
@@ -602,6 +723,60 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
     hipLaunchKernelGGL(energy_assemble_kernel, dim3((nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->e_direct,
                        c->e_exch, c->e_one, nw);
   });
+}
+
+// Dense same-spin exchange block of Vakbl from the half-rotated vectors
+// (the exchange half of the Vakbl construction the reference's input holds,
+// initialize_serial.hpp:132-164):  Vx_s[(a,k)][(b,l)] = sum_n Lt_s(n,a,l) Lt_s(n,b,k)
+// as N*N batched GEMMs over (a, b).  Lt64 is FP64 [2][K][NM].
+template <class OT> int build_vx(afqmc_ctx* c, const zd* Lt64)
+{
+  const long M = c->M, N = c->N, K = c->K, NM = N * M;
+  for (int sp = 0; sp < 2; sp++) {
+    ZGemmArgs a = gargs();
+    a.A = Lt64 + sp * K * NM;  // A_ab(l, n) = Lt(n, a, l)
+    a.sam = 1;
+    a.sak = NM;
+    a.sa1 = M;
+    a.sa2 = 0;
+    a.B = Lt64 + sp * K * NM;  // B_ab(k, n) = Lt(n, b, k)
+    a.sbn = 1;
+    a.sbk = NM;
+    a.sb1 = 0;
+    a.sb2 = M;
+    a.bdiv = (int)N;
+    a.C = (OT*)c->Vx + 2 * (sp * NM * NM);  // C_ab(l, k) -> Vx[(a,k)][(b,l)]
+    a.scm = 1;
+    a.scn = NM;
+    a.sc1 = M * NM;
+    a.sc2 = M;
+    a.m = (int)M;
+    a.n = (int)M;
+    a.k = (int)K;
+    RET((zgemm<CP_F64, double, double, OT, false, false>(c, "build_vx", a, (int)(N * N))));
+  }
+  return AFQMC_OK;
+}
+
+// Build Vx when it fits in device memory with room to spare; otherwise the
+// energy uses the Cholesky form.
+template <class ZT> int maybe_vx(afqmc_ctx* c, const zd* Lt64)
+{
+  if (!c->allow_vx) return AFQMC_OK;
+  const long NM = (long)c->N * c->M, nw = c->nw;
+  const size_t bytes = sizeof(ZT) * 2 * (size_t)NM * NM;
+  size_t fr = 0, tot = 0;
+  HIPCHK(c, hipMemGetInfo(&fr, &tot));
+  const size_t need = bytes + sizeof(ZT) * (size_t)nw * NM + ((size_t)8 << 30);
+  if (need > fr) return AFQMC_OK;
+  RET(dalloc_bytes(c, &c->Vx, bytes));
+  if ((long)c->ybuf_elems < nw * NM) {
+    dfree(c, c->Ybuf);
+    c->ybuf_elems = nw * NM;
+    RET(dalloc_bytes(c, &c->Ybuf, sizeof(ZT) * c->ybuf_elems));
+  }
+  if (sizeof(ZT) == sizeof(zd)) return build_vx<double>(c, Lt64);
+  return build_vx<float>(c, Lt64);
 }
 
 int do_energy(afqmc_ctx* c, double* eav_out)
@@ -706,6 +881,7 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   if (nmo <= 0 || naea <= 0 || naea > nmo || nchol <= 0 || nwalk <= 0) return AFQMC_ERR_SHAPE;
   if (mode < AFQMC_FP64_CSR || mode > AFQMC_MIXED_BF16) return AFQMC_ERR_ARG;
   if (naea > 128) return AFQMC_ERR_UNSUPPORTED;
+  if (lq_lds_bytes(nmo) > 160 * 1024) return AFQMC_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AFQMC_ERR_HIP;
   afqmc_ctx* c = new afqmc_ctx;
@@ -719,6 +895,12 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
     delete c;
     return AFQMC_ERR_HIP;
   }
+  if (hipFuncSetAttribute((const void*)lq_blocked_kernel<kLqNB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lq_lds_bytes(nmo)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AFQMC_ERR_HIP;
+  }
   const long M = nmo, N = naea, K = nchol, nw = nwalk;
   const long MN = M * N, MM = M * M;
   const size_t zs = c->mixed ? sizeof(zf) : sizeof(zd);
@@ -729,6 +911,7 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   chk(dalloc(c, &c->W, nw * 2 * MN));
   chk(dalloc(c, &c->Wd, nw * 8));
   chk(dalloc(c, &c->T1, nw * 2 * N * N));
+  chk(dalloc(c, &c->dcache, nw * 2));
   chk(dalloc_bytes(c, &c->G, nw * 2 * MN * zs));
   chk(dalloc(c, &c->vbias, nw * K));
   chk(dalloc_bytes(c, &c->X, nw * K * zs));
@@ -783,6 +966,17 @@ int afqmc_sync(afqmc_ctx* c)
   if (!c) return AFQMC_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return AFQMC_OK;
+}
+
+int afqmc_set_option(afqmc_ctx* c, int option, double value)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  if (option == AFQMC_OPT_DENSE_EXCHANGE) {
+    if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
+    c->allow_vx = value != 0.0;
+    return AFQMC_OK;
+  }
+  return fail(c, AFQMC_ERR_ARG, "unknown option");
 }
 
 int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* trial_beta, const double* propg1,
@@ -907,13 +1101,16 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Lt64);
     } else {
-      c->Lt = Lt64;  // Cholesky-form energy
+      c->Lt = Lt64;  // two-body energy from the vectors
       RET(dalloc(c, &c->Gtmp, 2 * NM * c->nw));
+      RET(maybe_vx<zd>(c, Lt64));
     }
   } else {
     dfree(c, rp);
     dfree(c, col);
     dfree(c, val);
+    if (c->mixed) RET(maybe_vx<zf>(c, Lt64));
+    else RET(maybe_vx<zd>(c, Lt64));
     if (c->mixed) {
       zf* Ld32 = nullptr;
       zf* Lt32 = nullptr;
@@ -957,6 +1154,7 @@ int afqmc_init_walkers(afqmc_ctx* c)
   if (!c) return AFQMC_ERR_ARG;
   if (!c->have_system) return fail(c, AFQMC_ERR_STATE, "upload a system first");
   const long MN = (long)c->M * c->N;
+  c->t1inv_valid = false;
   long tot = std::max<long>((long)c->nw * 2 * MN, (long)c->nw * 8);
   return launch(c, "init_walkers", [&]() {
     hipLaunchKernelGGL(init_walkers_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->W, c->Wd,
@@ -1005,6 +1203,7 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
     return AFQMC_OK;
   };
   const bool d = dense_layout(c);
+  if (to_device && buffer == AFQMC_BUF_W) c->t1inv_valid = false;
   switch (buffer) {
   case AFQMC_BUF_W: return raw(c->W, sizeof(zd) * nw * 2 * MN);
   case AFQMC_BUF_WDATA: return raw(c->Wd, sizeof(zd) * nw * 8);
@@ -1169,6 +1368,23 @@ int afqmc_run(afqmc_ctx* c, long first, int nsteps, int nsubsteps, int northo, d
   return check_flags(c);
 }
 
+int afqmc_tune(afqmc_ctx* c, double dt)
+{
+  NEED_SYSTEM(c);
+  if (!c->xstage || c->nstage < 1) return fail(c, AFQMC_ERR_STATE, "stage fields before tuning");
+  c->tuning = true;
+  int r = substep_body(c, c->xstage, 1, dt);
+  if (r == AFQMC_OK) r = do_mdm(c);
+  if (r == AFQMC_OK) r = do_energy(c, nullptr);
+  c->tuning = false;
+  c->t1inv_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->eshift, 0, sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return r;
+}
+
 int afqmc_use_graphs(afqmc_ctx* c, int enable)
 {
   if (!c) return AFQMC_ERR_ARG;
@@ -1227,6 +1443,8 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
                            int ldc)
 {
   if (!alpha || !beta || !A || !B || !Cm || m < 0 || n < 0 || k < 0) return AFQMC_ERR_ARG;
+  const int tile = (precision >> 8) & 0xff;
+  precision &= 0xff;
   if ((opA != 'N' && opA != 'T') || (opB != 'N' && opB != 'T') || precision < 0 || precision > 2)
     return AFQMC_ERR_ARG;
   const long ar = (opA == 'N') ? m : k, ac = (opA == 'N') ? k : m;
@@ -1235,6 +1453,7 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
   if (m == 0 || n == 0) return AFQMC_OK;
   afqmc_ctx tmp;
   afqmc_ctx* c = &tmp;
+  c->force_cfg = tile;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return AFQMC_ERR_HIP;
   zd *dA = nullptr, *dB = nullptr, *dC = nullptr;
   int r = AFQMC_OK;

@@ -44,6 +44,8 @@ struct ZGemmArgs {
   // null when only D is wanted.
   void* D;
   const void* F;
+  // tile grid: blockIdx.x runs over MT*NT*batch tiles (filled by the launcher)
+  int MT, NT;
 };
 
 template <int CP> struct LdsT;
@@ -92,8 +94,19 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int g = lane >> 4, r = lane & 15, s = r & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BNC;
-  const int b = blockIdx.z;
+  // XCD-aware tile order: consecutive workgroup ids go round-robin over the 8
+  // XCDs, so id -> (id % 8) * (T / 8) + id / 8 gives every XCD a contiguous
+  // run of tiles; within a run the M tile varies fastest, then the N tile,
+  // then the batch item, so tiles sharing a B tile or a batch item share an L2.
+  long lin = blockIdx.x;
+  {
+    const long T = gridDim.x, per = T / 8;
+    if (per > 0 && lin < per * 8) lin = (lin % 8) * per + lin / 8;
+  }
+  const int mt = (int)(lin % p.MT);
+  const long rest = lin / p.MT;
+  const int m0 = mt * BM, n0 = (int)(rest % p.NT) * BNC;
+  const int b = (int)(rest / p.NT);
   const long bq = b / p.bdiv, br_ = b % p.bdiv;
   const AZ* __restrict__ A = (const AZ*)p.A + bq * p.sa1 + br_ * p.sa2;
   const BZ* __restrict__ B = (const BZ*)p.B + bq * p.sb1 + br_ * p.sb2;

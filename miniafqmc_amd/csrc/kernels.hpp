@@ -13,12 +13,17 @@
 // match getrf; the inverse equals getri's up to rounding.
 //
 // Thread (i, cg) of the NMAX*CG threads holds row i, columns cg + CG*t of X in
-// registers.  Rows >= N are padded with the identity.
-// batch b: input T + b*mstride; det written to det[(b/2)*dq + (b%2)*dr].
+// registers (NMAX/CG complex values).  Rows >= N are padded with the
+// identity.  Per elimination step: holders of column k publish |X(i,k)|, every
+// wave finds the pivot row p, rows p and k and column k go through LDS, and
+// each thread updates its own entries.
+// batch b: input T + b*mstride; det written to det[(b/2)*dq + (b%2)*dr] and
+// to dcache[b] when dcache != null.
 template <int NMAX, int CG>
 __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
                                                                zd* __restrict__ det, long dq, long dr, int N,
-                                                               long mstride, int* __restrict__ info)
+                                                               long mstride, int* __restrict__ info,
+                                                               zd* __restrict__ dcache)
 {
   constexpr int NC = NMAX / CG;
   __shared__ double red[NMAX];
@@ -40,17 +45,17 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
   bool singular = false;
   for (int k = 0; k < N; k++) {
     const int tk = k / CG, ck = k % CG;
-    // pivot search over column k, rows >= k
+    zd xk{0, 0};  // this thread's X(i,k) when it holds column k
     if (cg == ck) {
-      zd v{0, 0};
 #pragma unroll
       for (int t = 0; t < NC; t++)
-        if (t == tk) v = x[t];
-      red[i] = (i >= k) ? zcabs1(v) : -1.0;
+        if (t == tk) xk = x[t];
+      red[i] = (i >= k) ? zcabs1(xk) : -1.0;
     }
     __syncthreads();
     double bv = -2.0;
     int bi = 0;
+#pragma unroll
     for (int q = lane; q < NMAX; q += 64) {
       double v = red[q];
       if (v > bv) { bv = v; bi = q; }
@@ -66,17 +71,11 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
 #pragma unroll
       for (int t = 0; t < NC; t++) prow[cg + CG * t] = x[t];
     }
-    if (i == k) {
+    if (i == k && p != k) {
 #pragma unroll
       for (int t = 0; t < NC; t++) oldk[cg + CG * t] = x[t];
     }
-    if (cg == ck) {
-      zd v{0, 0};
-#pragma unroll
-      for (int t = 0; t < NC; t++)
-        if (t == tk) v = x[t];
-      colk[i] = v;
-    }
+    if (cg == ck) colk[i] = xk;
     if (tid == 0) perm[k] = p;
     __syncthreads();
     const zd piv = prow[k];
@@ -87,25 +86,37 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
     }
     dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
     const zd ipv = zdiv(zd{1.0, 0.0}, piv);
-    const bool swapped_in = (i == p && p != k);
-    const zd f = swapped_in ? colk[k] : colk[i];
+    if (i == k) {
+      // new row k = pivot row / piv; column k holds 1/piv
 #pragma unroll
-    for (int t = 0; t < NC; t++) {
-      int j = cg + CG * t;
-      zd pr = (j == k) ? zd{1.0, 0.0} : prow[j];
-      zd sr = zmul(pr, ipv);  // scaled pivot row; column k holds 1/piv
-      if (i == k) {
-        x[t] = sr;
-      } else {
-        zd cur = swapped_in ? oldk[j] : x[t];
-        if (j == k) cur = zd{0.0, 0.0};
-        x[t] = zsub(cur, zmul(f, sr));
+      for (int t = 0; t < NC; t++) {
+        int j = cg + CG * t;
+        x[t] = (j == k) ? ipv : zmul(prow[j], ipv);
+      }
+    } else {
+      const bool swapped_in = (i == p);  // p != k here since i != k
+      if (swapped_in) {
+#pragma unroll
+        for (int t = 0; t < NC; t++) x[t] = oldk[cg + CG * t];
+      }
+      const zd f = swapped_in ? colk[k] : colk[i];
+      const zd gq = zmul(zd{-f.re, -f.im}, ipv);  // -X(i,k)/piv
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        const zd pr = prow[j];
+        const bool kk = (j == k);
+        const double xr = kk ? 0.0 : x[t].re, xi = kk ? 0.0 : x[t].im;
+        const double pr_r = kk ? 1.0 : pr.re, pr_i = kk ? 0.0 : pr.im;
+        x[t].re = xr + gq.re * pr_r - gq.im * pr_i;
+        x[t].im = xi + gq.re * pr_i + gq.im * pr_r;
       }
     }
   }
   if (tid == 0) {
     if (singular && info) atomicAdd(info, 1);
     det[(b / 2) * dq + (b % 2) * dr] = dt;
+    if (dcache) dcache[b] = dt;
   }
   if (!Tinv) return;
   if (singular) {
@@ -119,6 +130,7 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
     return;
   }
   // undo the row interchanges as column interchanges in reverse order
+  __syncthreads();
   if (tid == 0) {
     for (int c = 0; c < N; c++) dest[c] = c;  // dest used as colmap first
     for (int k = N - 1; k >= 0; k--) {
@@ -137,6 +149,14 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
     int j = cg + CG * t;
     if (i < N && j < N) Tinv[b * mstride + (long)perm[j] * N + i] = x[t];
   }
+}
+
+// W_data[w][2+s] = dcache[2w+s] (determinants cached by the last inverse of
+// the same walkers)
+__global__ void det_restore_kernel(zd* __restrict__ Wd, const zd* __restrict__ dcache, int nw)
+{
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < 2 * nw) Wd[(long)(b / 2) * 8 + 2 + (b % 2)] = dcache[b];
 }
 
 // ---------------------------------------------------------------------------
@@ -325,139 +345,294 @@ __global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, con
 
 // ---------------------------------------------------------------------------
 // Householder LQ with LAPACK conventions on the column-major view A = W^T
-// (N x M) of each W[w][s] ([M][N] row-major): zgelq2 followed by zungl2
-// (ma::gelqf / ma::glq, src/Numerics/ma_lapack.hpp:134-221; called by
-// afqmc_sys::orthogonalize, src/AFQMC/afqmc_sys.hpp:175-207).  A(i, c) is
-// W[c][i]: row i of A is column i of W.  One block per matrix; reflectors
-// applied with a block-wide GEMV + rank-1 update through L2.
-__device__ inline double block_sum(double v, double* sh)
+// (N x M) of each W[w][s] ([M][N] row-major): the blocked zgelqf followed by
+// the blocked zunglq (ma::gelqf / ma::glq, src/Numerics/ma_lapack.hpp:134-221;
+// called by afqmc_sys::orthogonalize, src/AFQMC/afqmc_sys.hpp:175-207).
+// A(i, c) is W[c][i]: row i of A is column i of W.  One block of 256 threads
+// per matrix.  Panels of NB rows are factorised in LDS (zgelq2 / zungl2);
+// the rows below a panel get the block reflector I - V^H T V (zlarft
+// 'Forward','Rowwise' + zlarfb 'Right') in two passes through L2.
+__device__ inline zd wave_zsum(zd v)
 {
-  // 256 threads
+  for (int o = 32; o >= 1; o >>= 1) {
+    v.re += __shfl_xor(v.re, o);
+    v.im += __shfl_xor(v.im, o);
+  }
+  return v;
+}
+__device__ inline double block_dsum(double v, double* sh)
+{
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) sh[wv] = v;
   __syncthreads();
-  double t = sh[0] + sh[1] + sh[2] + sh[3];
-  return t;
+  return sh[0] + sh[1] + sh[2] + sh[3];
 }
 
-__global__ __launch_bounds__(256) void lq_kernel(zd* __restrict__ Wall, int M, int N)
+// V(r, c) of a panel held in LDS rows P[r*LP + c]: 1 on the diagonal, the
+// stored tail right of it, 0 left of it.
+__device__ inline zd vpanel(const zd* P, int LP, int r, int c)
 {
-  constexpr int NTH = 256;
-  extern __shared__ zd tau[];  // [N]
-  __shared__ zd wsum[2 * 128]; // partial reductions of w = A v over two column halves
+  if (c == r) return zd{1.0, 0.0};
+  if (c < r) return zd{0.0, 0.0};
+  return P[r * LP + c];
+}
+
+// zlarfg on panel row r (already conjugated), LAPACK sign rule.
+__device__ inline bool householder_row(zd* P, int LP, int r, double* sh, zd& alpha, zd& taui, double& beta)
+{
+  const int tid = threadIdx.x;
+  double ss = 0.0;
+  for (int c = r + 1 + tid; c < LP; c += 256) ss += P[r * LP + c].re * P[r * LP + c].re + P[r * LP + c].im * P[r * LP + c].im;
+  const double xn2 = block_dsum(ss, sh);
+  alpha = P[r * LP + r];
+  const bool apply = !(xn2 == 0.0 && alpha.im == 0.0);
+  taui = zd{0, 0};
+  beta = 0.0;
+  if (apply) {
+    const double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+    beta = -copysign(nrm, alpha.re);
+    taui = zd{(beta - alpha.re) / beta, -alpha.im / beta};
+    const zd scal = zdiv(zd{1.0, 0.0}, zd{alpha.re - beta, alpha.im});
+    for (int c = r + 1 + tid; c < LP; c += 256) P[r * LP + c] = zmul(scal, P[r * LP + c]);
+  }
+  __syncthreads();
+  return apply;
+}
+
+// apply I - t v v^H (v = panel row r with unit diagonal) from the right to
+// panel rows r+1..ib-1 over columns r..LP-1
+template <int NB>
+__device__ inline void panel_apply(zd* P, int LP, int r, int ib, zd t, zd* red)
+{
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  zd acc[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) acc[q] = zd{0, 0};
+  for (int c = r + tid; c < LP; c += 256) {
+    const zd vc = (c == r) ? zd{1.0, 0.0} : P[r * LP + c];
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (q > r && q < ib) acc[q] = zadd(acc[q], zmul(P[q * LP + c], vc));
+  }
+#pragma unroll
+  for (int q = 0; q < NB; q++) {
+    zd s = wave_zsum(acc[q]);
+    if (lane == 0) red[wv * NB + q] = s;
+  }
+  __syncthreads();
+  zd wq[NB];
+#pragma unroll
+  for (int q = 0; q < NB; q++) wq[q] = zmul(t, zadd(zadd(red[q], red[NB + q]), zadd(red[2 * NB + q], red[3 * NB + q])));
+  for (int c = r + tid; c < LP; c += 256) {
+    const zd vcc = zconj((c == r) ? zd{1.0, 0.0} : P[r * LP + c]);
+#pragma unroll
+    for (int q = 0; q < NB; q++)
+      if (q > r && q < ib) P[q * LP + c] = zsub(P[q * LP + c], zmul(wq[q], vcc));
+  }
+  __syncthreads();
+}
+
+// T (ib x ib, upper) of zlarft('Forward','Rowwise') from the panel and taus.
+// The ib(ib-1)/2 dot products V(j,:) V(i,:)^H (j < i, columns >= i) are
+// reduced block-wide, then one thread runs the triangular recursion.
+template <int NB>
+__device__ inline void form_T(const zd* P, int LP, int ib, const zd* tau, zd* T, zd* red)
+{
+  constexpr int NP = NB * (NB - 1) / 2;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  zd acc[NP];
+#pragma unroll
+  for (int e = 0; e < NP; e++) acc[e] = zd{0, 0};
+  for (int c = tid; c < LP; c += 256) {
+    zd pj[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) pj[j] = (j < ib) ? P[j * LP + c] : zd{0, 0};
+#pragma unroll
+    for (int i = 1; i < NB; i++) {
+      if (i >= ib || c < i) continue;
+      const zd vi = zconj((c == i) ? zd{1.0, 0.0} : pj[i]);
+#pragma unroll
+      for (int j = 0; j < i; j++) acc[i * (i - 1) / 2 + j] = zadd(acc[i * (i - 1) / 2 + j], zmul(pj[j], vi));
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < NP; e++) {
+    zd s = wave_zsum(acc[e]);
+    if (lane == 0) red[wv * NP + e] = s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < ib; i++) {
+      const zd ti = tau[i];
+      if (ti.re == 0.0 && ti.im == 0.0) {
+        for (int j = 0; j <= i; j++) T[j * NB + i] = zd{0, 0};
+        continue;
+      }
+      // T(0:i-1, i) = -tau_i V(0:i-1, :) V(i, :)^H, then T(0:i-1,0:i-1) times it (in place, ascending rows)
+      for (int j = 0; j < i; j++) {
+        const int e = i * (i - 1) / 2 + j;
+        zd d = zadd(zadd(red[e], red[NP + e]), zadd(red[2 * NP + e], red[3 * NP + e]));
+        T[j * NB + i] = zmul(zd{-ti.re, -ti.im}, d);
+      }
+      for (int j = 0; j < i; j++) {
+        zd s{0, 0};
+        for (int l = j; l < i; l++) s = zadd(s, zmul(T[j * NB + l], T[l * NB + i]));
+        T[j * NB + i] = s;
+      }
+      T[i * NB + i] = ti;
+    }
+  }
+  __syncthreads();
+}
+
+// C <- C (I - V^H op(T) V) for rows rb..N-1 of A, columns c0..M-1 (panel
+// columns are local c = global - c0).  conjT: op(T) = T^H (zunglq), else T.
+template <int NB>
+__device__ inline void trailing_update(zd* W, int N, int rb, int c0, int LP, const zd* P, int ib, const zd* T,
+                                       bool conjT, zd* W1)
+{
+  const int tid = threadIdx.x;
+  const int nr = N - rb;
+  for (int q0 = 0; q0 < nr; q0 += 128) {
+    const int ql = tid & 127, half = tid >> 7;
+    const int q = q0 + ql;
+    zd acc[NB];
+#pragma unroll
+    for (int r = 0; r < NB; r++) acc[r] = zd{0, 0};
+    if (q < nr) {
+      for (int c = half; c < LP; c += 2) {
+        const zd cv = W[(long)(c0 + c) * N + rb + q];
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+          if (r < ib) acc[r] = zadd(acc[r], zmul(cv, zconj(vpanel(P, LP, r, c))));
+      }
+    }
+    if (half == 1) {
+#pragma unroll
+      for (int r = 0; r < NB; r++) W1[ql * NB + r] = acc[r];
+    }
+    __syncthreads();
+    zd w2[NB];
+    if (half == 0) {
+#pragma unroll
+      for (int r = 0; r < NB; r++) acc[r] = zadd(acc[r], W1[ql * NB + r]);
+      // W2 = W1 op(T)
+#pragma unroll
+      for (int r = 0; r < NB; r++) {
+        zd s{0, 0};
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          if (j >= ib || r >= ib) continue;
+          zd tjr = conjT ? ((r <= j) ? zconj(T[r * NB + j]) : zd{0, 0}) : ((j <= r) ? T[j * NB + r] : zd{0, 0});
+          s = zadd(s, zmul(acc[j], tjr));
+        }
+        w2[r] = s;
+      }
+    }
+    __syncthreads();
+    if (half == 0) {
+#pragma unroll
+      for (int r = 0; r < NB; r++) W1[ql * NB + r] = w2[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NB; r++) w2[r] = W1[ql * NB + r];
+    if (q < nr) {
+      for (int c = half; c < LP; c += 2) {
+        zd s{0, 0};
+#pragma unroll
+        for (int r = 0; r < NB; r++)
+          if (r < ib) s = zadd(s, zmul(w2[r], vpanel(P, LP, r, c)));
+        const long a = (long)(c0 + c) * N + rb + q;
+        W[a] = zsub(W[a], s);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void lq_blocked_kernel(zd* __restrict__ Wall, int M, int N)
+{
+  extern __shared__ zd sm[];
+  zd* P = sm;                  // [NB][M]
+  zd* W1 = P + NB * M;         // [128][NB]
+  zd* T = W1 + 128 * NB;       // [NB][NB]
+  zd* tau = T + NB * NB;       // [N] (N <= 128)
+  zd* red = tau + 128;         // [4][NB*NB]
   __shared__ double sh[8];
   zd* W = Wall + (long)blockIdx.x * M * N;
   const int tid = threadIdx.x;
-#define AIC(i, c) W[(long)(c) * N + (i)]
-  // ---- zgelq2 ----
-  for (int i = 0; i < N; i++) {
-    // conj row i (c >= i) and norm of A(i, i+1:M)
-    double ss = 0.0;
-    for (int c = i + tid; c < M; c += NTH) {
-      zd v = zconj(AIC(i, c));
-      AIC(i, c) = v;
-      if (c > i) ss += v.re * v.re + v.im * v.im;
-    }
-    double xn2 = block_sum(ss, sh);
-    __syncthreads();
-    zd alpha = AIC(i, i);
-    double xnorm = sqrt(xn2);
-    zd taui{0, 0}, scal{0, 0};
-    double beta = 0.0;
-    bool apply = !(xnorm == 0.0 && alpha.im == 0.0);
-    if (apply) {
-      double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
-      beta = -copysign(nrm, alpha.re);  // LAPACK: -sign(|(alpha,x)|, Re alpha)
-      taui = zd{(beta - alpha.re) / beta, -alpha.im / beta};
-      scal = zdiv(zd{1.0, 0.0}, zd{alpha.re - beta, alpha.im});
-    }
-    if (tid == 0) tau[i] = taui;
-    if (apply) {
-      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zmul(scal, AIC(i, c));
+  auto load_panel = [&](int i0, int ib, int LP) {
+    for (int e = tid; e < ib * LP; e += 256) {
+      const int r = e % ib, c = e / ib;
+      P[r * LP + c] = W[(long)(i0 + c) * N + i0 + r];
     }
     __syncthreads();
-    // apply H(i) from the right to rows r = i+1..N-1 over columns c = i..M-1 with v = [1, A(i,i+1:)]
-    if (apply && i < N - 1) {
-      const int nr = N - i - 1;
-      // w_r = sum_c A(r,c) v_c ; threads: r fastest, c split in NTH/nr... use 2D loop
-      for (int r0 = 0; r0 < nr; r0 += 128) {
-        const int rr = tid & 127, cp = tid >> 7;  // 2 column parts
-        const int r = i + 1 + r0 + rr;
-        zd acc{0, 0};
-        if (r0 + rr < nr) {
-          for (int c = i + cp; c < M; c += 2) {
-            zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
-            acc = zadd(acc, zmul(AIC(r, c), v));
-          }
-        }
-        wsum[cp * 128 + rr] = acc;
-        __syncthreads();
-        if (tid < 128 && r0 + tid < nr) wsum[tid] = zadd(wsum[tid], wsum[128 + tid]);
-        __syncthreads();
-        // A(r,c) -= tau * w_r * conj(v_c)
-        for (int e = tid; e < 128 * (M - i); e += NTH) {
-          int rl = e % 128, c = i + e / 128;
-          if (r0 + rl >= nr) continue;
-          int rA = i + 1 + r0 + rl;
-          zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
-          zd t = zmul(zmul(taui, wsum[rl]), zconj(v));
-          AIC(rA, c) = zsub(AIC(rA, c), t);
-        }
-        __syncthreads();
-      }
+  };
+  auto store_panel = [&](int i0, int ib, int LP) {
+    for (int e = tid; e < ib * LP; e += 256) {
+      const int r = e % ib, c = e / ib;
+      W[(long)(i0 + c) * N + i0 + r] = P[r * LP + c];
     }
-    if (tid == 0) AIC(i, i) = zd{beta, 0.0};
-    if (!apply && tid == 0) AIC(i, i) = alpha;
     __syncthreads();
-    for (int c = i + tid; c < M; c += NTH) AIC(i, c) = zconj(AIC(i, c));
-    __syncthreads();
-  }
-  // ---- zungl2 (k = N = m rows) ----
-  for (int i = N - 1; i >= 0; i--) {
-    const zd ti = tau[i];
-    if (i < M - 1) {
-      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zconj(AIC(i, c));
+  };
+  // ---- zgelqf ----
+  for (int i0 = 0; i0 < N; i0 += NB) {
+    const int ib = min(NB, N - i0), LP = M - i0;
+    load_panel(i0, ib, LP);
+    for (int r = 0; r < ib; r++) {
+      for (int c = r + tid; c < LP; c += 256) P[r * LP + c] = zconj(P[r * LP + c]);
       __syncthreads();
-      if (i < N - 1) {
-        const zd tc = zconj(ti);
-        const int nr = N - i - 1;
-        const bool nz = !(tc.re == 0.0 && tc.im == 0.0);
-        for (int r0 = 0; r0 < nr && nz; r0 += 128) {
-          const int rr = tid & 127, cp = tid >> 7;
-          const int r = i + 1 + r0 + rr;
-          zd acc{0, 0};
-          if (r0 + rr < nr) {
-            for (int c = i + cp; c < M; c += 2) {
-              zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
-              acc = zadd(acc, zmul(AIC(r, c), v));
-            }
-          }
-          wsum[cp * 128 + rr] = acc;
-          __syncthreads();
-          if (tid < 128 && r0 + tid < nr) wsum[tid] = zadd(wsum[tid], wsum[128 + tid]);
-          __syncthreads();
-          for (int e = tid; e < 128 * (M - i); e += NTH) {
-            int rl = e % 128, c = i + e / 128;
-            if (r0 + rl >= nr) continue;
-            int rA = i + 1 + r0 + rl;
-            zd v = (c == i) ? zd{1.0, 0.0} : AIC(i, c);
-            zd t = zmul(zmul(tc, wsum[rl]), zconj(v));
-            AIC(rA, c) = zsub(AIC(rA, c), t);
-          }
-          __syncthreads();
-        }
+      zd alpha, taui;
+      double beta;
+      const bool apply = householder_row(P, LP, r, sh, alpha, taui, beta);
+      if (tid == 0) tau[i0 + r] = taui;
+      if (apply && r < ib - 1) panel_apply<NB>(P, LP, r, ib, taui, red);
+      if (tid == 0) P[r * LP + r] = apply ? zd{beta, 0.0} : alpha;
+      __syncthreads();
+      for (int c = r + tid; c < LP; c += 256) P[r * LP + c] = zconj(P[r * LP + c]);
+      __syncthreads();
+    }
+    store_panel(i0, ib, LP);
+    if (i0 + ib < N) {
+      form_T<NB>(P, LP, ib, tau + i0, T, red);
+      trailing_update<NB>(W, N, i0 + ib, i0, LP, P, ib, T, false, W1);
+    }
+  }
+  // ---- zunglq (k = N reflectors, blocks from last to first) ----
+  const int last = ((N - 1) / NB) * NB;
+  for (int i0 = last; i0 >= 0; i0 -= NB) {
+    const int ib = min(NB, N - i0), LP = M - i0;
+    load_panel(i0, ib, LP);
+    if (i0 + ib < N) {
+      form_T<NB>(P, LP, ib, tau + i0, T, red);
+      trailing_update<NB>(W, N, i0 + ib, i0, LP, P, ib, T, true, W1);
+    }
+    // zungl2 on the panel rows
+    for (int r = ib - 1; r >= 0; r--) {
+      const zd ti = tau[i0 + r];
+      if (r < LP - 1) {
+        for (int c = r + 1 + tid; c < LP; c += 256) P[r * LP + c] = zconj(P[r * LP + c]);
+        __syncthreads();
+        if (r < ib - 1 && !(ti.re == 0.0 && ti.im == 0.0)) panel_apply<NB>(P, LP, r, ib, zconj(ti), red);
+        const zd mt{-ti.re, -ti.im};
+        for (int c = r + 1 + tid; c < LP; c += 256) P[r * LP + c] = zconj(zmul(mt, P[r * LP + c]));
       }
-      const zd mt{-ti.re, -ti.im};
-      for (int c = i + 1 + tid; c < M; c += NTH) AIC(i, c) = zconj(zmul(mt, AIC(i, c)));
+      __syncthreads();
+      if (tid == 0) P[r * LP + r] = zd{1.0 - ti.re, ti.im};
+      for (int c = tid; c < r; c += 256) P[r * LP + c] = zd{0, 0};
+      __syncthreads();
+    }
+    store_panel(i0, ib, LP);
+    for (int e = tid; e < ib * i0; e += 256) {
+      const int r = e % ib, c = e / ib;
+      W[(long)c * N + i0 + r] = zd{0, 0};
     }
     __syncthreads();
-    if (tid == 0) AIC(i, i) = zd{1.0 - ti.re, ti.im};
-    for (int c = tid; c < i; c += NTH) AIC(i, c) = zd{0, 0};
-    __syncthreads();
   }
-#undef AIC
 }
 
 // ---------------------------------------------------------------------------

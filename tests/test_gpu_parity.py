@@ -144,6 +144,41 @@ def test_phases_fp64_csr(c1):
     g.close()
 
 
+@pytest.mark.parametrize("M,N", [(100, 24), (300, 100), (512, 128)])
+def test_orthogonalize_sizes(require_gpu, M, N):
+    """Blocked Householder LQ (ragged panels, up to 512 x 128) vs the oracle's zgelq2/zungl2."""
+    say("orthogonalize", M, N)
+    s = m.make_system(M, N, 2, density=1.0, sigma_L=0.01, seed=M)
+    nw = 3
+    g = m.AfqmcSys(s, nw, "fp64_csr", dense_exchange=False)
+    r = np.random.default_rng(M)
+    W = r.standard_normal((nw, 2, M, N)) + 1j * r.standard_normal((nw, 2, M, N))
+    g.set(nat.BUF_W, W)
+    g.orthogonalize()
+    close(g.W, orc.orthogonalize(W), 1e-10, "orthogonalized walkers")
+    g.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64_dense", "mixed_fp32"])
+def test_energy_dense_exchange_equals_cholesky(c1, mode):
+    """The dense exchange block and the Cholesky form give the same E_L."""
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=9)
+    out = []
+    for vx in (True, False):
+        g = m.AfqmcSys(s, nw, mode, dense_exchange=vx)
+        g.set(nat.BUF_W, W)
+        wd = np.zeros((nw, 8), complex)
+        wd[:, 1] = 1
+        g.set(nat.BUF_WDATA, wd)
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        out.append(g.W_data[:, 0])
+        g.close()
+    close(out[0], out[1], 1e-12 if mode == "fp64_dense" else 1e-5, "E_L dense exchange vs Cholesky")
+
+
 def run_both(s, p, mode, nsteps=None, nw=None, cutoff=1e-6):
     nw = nw or p["nwalk"]
     ns = nsteps or p["nsteps"]
