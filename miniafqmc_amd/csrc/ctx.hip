@@ -19,6 +19,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -1378,6 +1379,8 @@ int afqmc_tune(afqmc_ctx* c, double dt)
   if (r == AFQMC_OK) r = do_energy(c, nullptr);
   c->tuning = false;
   c->t1inv_valid = false;
+  if (std::getenv("AFQMC_VERBOSE"))
+    for (auto& kv : c->tune) std::fprintf(stderr, "[afqmc] tile %-48s -> candidate %d\n", kv.first.c_str(), kv.second);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream));
   HIPCHK(c, hipMemsetAsync(c->eshift, 0, sizeof(double), c->stream));

@@ -84,8 +84,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   using LT = typename L::T;
   constexpr int LDK = BKC + L::PAD;
   constexpr int NPLANE = (CP == CP_BF16X3) ? 2 : 1;
-  __shared__ __align__(16) LT sA[NPLANE][BM * LDK];
-  __shared__ __align__(16) LT sB[NPLANE][BNC * LDK];
+  // two LDS stages (ping-pong): the next k-tile is stored while others compute
+  __shared__ __align__(16) LT sA[2][NPLANE][BM * LDK];
+  __shared__ __align__(16) LT sB[2][NPLANE][BNC * LDK];
   using AZ = typename ztype<AT>::type;
   using BZ = typename ztype<BT>::type;
   using ACC = typename std::conditional<CP == CP_F64, doublex4, floatx4>::type;
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       else zero_z(rb[i]);
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](int st) {
 #pragma unroll
     for (int i = 0; i < EA; i++) {
       int e = tid + i * NT;
@@ -153,15 +154,15 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int row, kc;
       if (AK) { row = e / BKC; kc = e % BKC; } else { kc = e / BM; row = e % BM; }
       if constexpr (CP == CP_F64) {
-        sA[0][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
+        sA[st][0][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
       } else if constexpr (CP == CP_F32) {
-        sA[0][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
+        sA[st][0][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
       } else {
         float xr = (float)ra[i].re, xi = (float)ra[i].im;
         unsigned short hr = f2bf(xr), hi = f2bf(xi);
         unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
-        sA[0][row * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
-        sA[1][row * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+        sA[st][0][row * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
+        sA[st][1][row * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
       }
     }
 #pragma unroll
@@ -171,26 +172,27 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int col, kc;
       if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
       if constexpr (CP == CP_F64) {
-        sB[0][col * LDK + kc] = zd{(double)rb[i].re, (double)rb[i].im};
+        sB[st][0][col * LDK + kc] = zd{(double)rb[i].re, (double)rb[i].im};
       } else if constexpr (CP == CP_F32) {
-        sB[0][col * LDK + kc] = zf{(float)rb[i].re, (float)rb[i].im};
+        sB[st][0][col * LDK + kc] = zf{(float)rb[i].re, (float)rb[i].im};
       } else {
         float xr = (float)rb[i].re, xi = (float)rb[i].im;
         unsigned short hr = f2bf(xr), hi = f2bf(xi);
         unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
-        sB[0][col * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
-        sB[1][col * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+        sB[st][0][col * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
+        sB[st][1][col * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
       }
     }
   };
 
   const int nk = (p.k + BKC - 1) / BKC;
   gload(0);
-  sstore();
+  sstore(0);
   __syncthreads();
   const int arow0 = wm * WTM + r;           // + bi*16
   const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
   for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BKC);
     if constexpr (CP == CP_F64 || CP == CP_F32) {
       using RT = typename std::conditional<CP == CP_F64, double, float>::type;
@@ -198,9 +200,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       for (int kq = 0; kq < BKC / 4; kq++) {
         LT a[TM], bv[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) a[i] = sA[0][(arow0 + i * 16) * LDK + kq * 4 + g];
+        for (int i = 0; i < TM; i++) a[i] = sA[cur][0][(arow0 + i * 16) * LDK + kq * 4 + g];
 #pragma unroll
-        for (int j = 0; j < TN; j++) bv[j] = sB[0][(bcol0 + j * 8) * LDK + kq * 4 + g];
+        for (int j = 0; j < TN; j++) bv[j] = sB[cur][0][(bcol0 + j * 8) * LDK + kq * 4 + g];
 #pragma unroll
         for (int j = 0; j < TN; j++) {
           RT b0 = s ? bv[j].im : bv[j].re;
@@ -223,13 +225,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         uint4 ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
         for (int i = 0; i < TM; i++) {
-          ah[i] = *(const uint4*)&sA[0][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
-          al[i] = *(const uint4*)&sA[1][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
+          ah[i] = *(const uint4*)&sA[cur][0][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
+          al[i] = *(const uint4*)&sA[cur][1][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
         }
 #pragma unroll
         for (int j = 0; j < TN; j++) {
-          uint4 h = *(const uint4*)&sB[0][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
-          uint4 l = *(const uint4*)&sB[1][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
+          uint4 h = *(const uint4*)&sB[cur][0][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
+          uint4 l = *(const uint4*)&sB[cur][1][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
           if (s == 0) {
             h.x ^= 0x80000000u; h.y ^= 0x80000000u; h.z ^= 0x80000000u; h.w ^= 0x80000000u;
             l.x ^= 0x80000000u; l.y ^= 0x80000000u; l.z ^= 0x80000000u; l.w ^= 0x80000000u;
@@ -257,11 +259,10 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         }
       }
     }
-    if (kt + 1 < nk) {
-      __syncthreads();
-      sstore();
-      __syncthreads();
-    }
+    // stage cur^1 was last read in iteration kt-1, which every thread has
+    // finished (barrier below), so it can be refilled now
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
   }
 
   // epilogue: lanes (2c, 2c+1) hold Re and Im of complex column c

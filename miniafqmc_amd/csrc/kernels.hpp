@@ -489,24 +489,36 @@ __device__ inline void form_T(const zd* P, int LP, int ib, const zd* tau, zd* T,
 
 // C <- C (I - V^H op(T) V) for rows rb..N-1 of A, columns c0..M-1 (panel
 // columns are local c = global - c0).  conjT: op(T) = T^H (zunglq), else T.
+// Threads: 128 rows x 2 column halves; each thread keeps UNR independent
+// loads in flight per iteration.
 template <int NB>
 __device__ inline void trailing_update(zd* W, int N, int rb, int c0, int LP, const zd* P, int ib, const zd* T,
                                        bool conjT, zd* W1)
 {
+  constexpr int UNR = 8;
   const int tid = threadIdx.x;
   const int nr = N - rb;
+  const int half = tid >> 7;
+  const int hl = (LP + 1) / 2;           // columns of half 0
+  const int cb = half ? hl : 0, ce = half ? LP : hl;
   for (int q0 = 0; q0 < nr; q0 += 128) {
-    const int ql = tid & 127, half = tid >> 7;
+    const int ql = tid & 127;
     const int q = q0 + ql;
+    const bool active = q < nr;
     zd acc[NB];
 #pragma unroll
     for (int r = 0; r < NB; r++) acc[r] = zd{0, 0};
-    if (q < nr) {
-      for (int c = half; c < LP; c += 2) {
-        const zd cv = W[(long)(c0 + c) * N + rb + q];
+    for (int c = cb; c < ce; c += UNR) {
+      zd cv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; u++)
+        cv[u] = (active && c + u < ce) ? W[(long)(c0 + c + u) * N + rb + q] : zd{0, 0};
+#pragma unroll
+      for (int u = 0; u < UNR; u++) {
+        if (c + u >= ce) break;
 #pragma unroll
         for (int r = 0; r < NB; r++)
-          if (r < ib) acc[r] = zadd(acc[r], zmul(cv, zconj(vpanel(P, LP, r, c))));
+          if (r < ib) acc[r] = zadd(acc[r], zmul(cv[u], zconj(vpanel(P, LP, r, c + u))));
       }
     }
     if (half == 1) {
@@ -539,14 +551,20 @@ __device__ inline void trailing_update(zd* W, int N, int rb, int c0, int LP, con
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < NB; r++) w2[r] = W1[ql * NB + r];
-    if (q < nr) {
-      for (int c = half; c < LP; c += 2) {
-        zd s{0, 0};
+    if (active) {
+      for (int c = cb; c < ce; c += UNR) {
+        zd cv[UNR];
 #pragma unroll
-        for (int r = 0; r < NB; r++)
-          if (r < ib) s = zadd(s, zmul(w2[r], vpanel(P, LP, r, c)));
-        const long a = (long)(c0 + c) * N + rb + q;
-        W[a] = zsub(W[a], s);
+        for (int u = 0; u < UNR; u++) cv[u] = (c + u < ce) ? W[(long)(c0 + c + u) * N + rb + q] : zd{0, 0};
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+          if (c + u >= ce) break;
+          zd s{0, 0};
+#pragma unroll
+          for (int r = 0; r < NB; r++)
+            if (r < ib) s = zadd(s, zmul(w2[r], vpanel(P, LP, r, c + u)));
+          W[(long)(c0 + c + u) * N + rb + q] = zsub(cv[u], s);
+        }
       }
     }
     __syncthreads();
