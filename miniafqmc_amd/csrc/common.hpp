@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -40,13 +41,41 @@ __device__ inline unsigned short f2bf(float x)
 }
 __device__ inline float bf2f(unsigned short b) { return __uint_as_float(((uint32_t)b) << 16); }
 
+// Split-BF16 complex: x = hi + lo per component, hi = bf16(x),
+// lo = bf16(x - hi); each half packed as (re | im << 16).  8 bytes like a
+// complex float, so buffers keep their size in either format.
+struct zb {
+  uint32_t hi, lo;
+};
+struct bfs {};  // element tag of zb arrays (one complex per zb)
+
 template <class T> struct ztype;
+template <> struct ztype<bfs> {
+  using type = zb;
+};
 template <> struct ztype<double> {
   using type = zd;
 };
 template <> struct ztype<float> {
   using type = zf;
 };
+
+__device__ inline zb split_z(float re, float im)
+{
+  unsigned short hr = f2bf(re), hi = f2bf(im);
+  unsigned short lr = f2bf(re - bf2f(hr)), li = f2bf(im - bf2f(hi));
+  return zb{(uint32_t)hr | ((uint32_t)hi << 16), (uint32_t)lr | ((uint32_t)li << 16)};
+}
+__device__ inline zd to_zd(zd v) { return v; }
+__device__ inline zd to_zd(zf v) { return zd{(double)v.re, (double)v.im}; }
+__device__ inline zd to_zd(zb v)
+{
+  return zd{(double)bf2f((unsigned short)(v.hi & 0xffff)) + (double)bf2f((unsigned short)(v.lo & 0xffff)),
+            (double)bf2f((unsigned short)(v.hi >> 16)) + (double)bf2f((unsigned short)(v.lo >> 16))};
+}
+__device__ inline void from_zd(zd v, zd& o) { o = v; }
+__device__ inline void from_zd(zd v, zf& o) { o = zf{(float)v.re, (float)v.im}; }
+__device__ inline void from_zd(zd v, zb& o) { o = split_z((float)v.re, (float)v.im); }
 
 __device__ inline zd load_z(const zd* p) { return *p; }
 __device__ inline zd load_z(const zf* p)

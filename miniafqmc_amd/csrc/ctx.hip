@@ -45,13 +45,14 @@ struct afqmc_ctx {
   std::string err;
   bool have_system = false;
   bool mixed = false;   // FP32 or split-BF16 contractions
+  bool bsplit = false;  // split-BF16: G, X, vHS, Taylor terms, Ld, Lt, Vx, Propg1 stored pre-split (zb)
   double dt_sys = 0.0;  // dt the system was scaled with
   std::vector<void*> allocs;
 
   // system
   zd* trial = nullptr;   // [2][M][N] conj(Psi)
   zd* propg = nullptr;   // [M][M]
-  zf* propg32 = nullptr; // [M][M] (mixed)
+  void* propg32 = nullptr; // [M][M] in the contraction format (mixed: zf, or zb when split)
   zd* haj = nullptr;     // [2N][M]
   int* sp_rp = nullptr;
   int* sp_col = nullptr;
@@ -215,6 +216,13 @@ void collect_events(afqmc_ctx* c)
     c->evpool.push_back(p.b);
   }
   c->pending.clear();
+}
+
+// element pointer advanced by `off` complex numbers (element tags double,
+// float or bfs, whose complex types are zd, zf, zb)
+template <class T> inline const void* zadv(const void* p, long off)
+{
+  return (const char*)p + off * (long)sizeof(typename ztype<T>::type);
 }
 
 inline int nblk(long n, int t) { return (int)std::min<long>((n + t - 1) / t, 65535L * 8); }
@@ -419,6 +427,7 @@ int do_mdm(afqmc_ctx* c)
   a.scn = 1;
   a.sc1 = 2L * N * M;
   a.sc2 = (long)N * M;
+  if (c->bsplit) return zgemm<CP_F64, double, double, bfs, true, true>(c, "gc_gemm", a, 2 * nw);
   if (c->mixed) return zgemm<CP_F64, double, double, float, true, true>(c, "gc_gemm", a, 2 * nw);
   return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
 }
@@ -440,10 +449,10 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, zd* out, const char* name
   const long NM = (long)c->N * c->M;
   for (int sp = 0; sp < 2; sp++) {
     ZGemmArgs a = gargs();
-    a.A = (const GT*)Gp + 2 * sp * NM;
+    a.A = zadv<GT>(Gp, sp * NM);
     a.sam = 2 * NM;
     a.sak = 1;
-    a.B = (const LT_*)c->Lt + 2 * (sp * (long)c->K * NM);
+    a.B = zadv<LT_>(c->Lt, sp * (long)c->K * NM);
     a.sbn = NM;
     a.sbk = 1;
     a.C = out;
@@ -471,7 +480,7 @@ int do_vbias(afqmc_ctx* c)
   }
   if (c->mode == AFQMC_FP64_DENSE) return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->vbias, "vbias_gemm");
   if (c->mode == AFQMC_MIXED_FP32) return dense_contract_vbias<CP_F32, float, float>(c, c->G, c->vbias, "vbias_gemm");
-  return dense_contract_vbias<CP_BF16X3, float, float>(c, c->G, c->vbias, "vbias_gemm");
+  return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->vbias, "vbias_gemm");
 }
 
 int do_fields(afqmc_ctx* c, const double* xdev)
@@ -487,6 +496,11 @@ int do_fields(afqmc_ctx* c, const double* xdev)
     return launch(c, "fields", [&]() {
       hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
                          (zd*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+    });
+  if (c->bsplit)
+    return launch(c, "fields", [&]() {
+      hipLaunchKernelGGL((fields_kernel<zd, zb>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
+                         (zb*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
     });
   return launch(c, "fields", [&]() {
     hipLaunchKernelGGL((fields_kernel<zd, zf>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
@@ -527,14 +541,14 @@ int do_vhs(afqmc_ctx* c)
   a.k = K;
   if (c->mode == AFQMC_FP64_DENSE) return zgemm<CP_F64, double, double, double, true, false>(c, "vhs_gemm", a, 1);
   if (c->mode == AFQMC_MIXED_FP32) return zgemm<CP_F32, float, float, float, true, false>(c, "vhs_gemm", a, 1);
-  return zgemm<CP_BF16X3, float, float, float, true, false>(c, "vhs_gemm", a, 1);
+  return zgemm<CP_BF16X3, bfs, bfs, bfs, true, false>(c, "vhs_gemm", a, 1);
 }
 
 // W <- P * T6(V) * P * W per walker and spin (afqmc_sys.hpp:146-173), T6 the
 // order-6 Taylor polynomial of exp(i V) applied as in apply_expM
 // (vHS.hpp:61-91): T_n = (i/n) V T_{n-1}, S += T_n.
-template <int CP, class ST, class PT, class VT>
-int propagate_chain(afqmc_ctx* c, const PT* P, const VT* V)
+template <int CP, class ST, class TT, class PT, class VT>
+int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
 {
   const int M = c->M, N = c->N, nw = c->nw;
   const long MN = (long)M * N, MM = (long)M * M;
@@ -577,7 +591,8 @@ int propagate_chain(afqmc_ctx* c, const PT* P, const VT* V)
     a.ai = 1.0 / (double)n;
     a.D = c->Sacc;
     a.F = (n == 1) ? c->S : c->Sacc;
-    RET((zgemm<CP, VT, ST, ST, true, false>(c, "prop_taylor", a, 2 * nw)));
+    if (n == 1) RET((zgemm<CP, VT, ST, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
+    else RET((zgemm<CP, VT, TT, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
     prev = bufs[n & 1];
   }
   // W = P * S
@@ -594,12 +609,12 @@ int propagate_chain(afqmc_ctx* c, const PT* P, const VT* V)
 int do_propagate(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
-  if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHSw);
+  if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHSw);
   if (c->mode == AFQMC_FP64_DENSE)
-    return propagate_chain<CP_F64, double, double, double>(c, (const double*)c->propg, (const double*)c->vHS);
+    return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHS);
   if (c->mode == AFQMC_MIXED_FP32)
-    return propagate_chain<CP_F32, float, float, float>(c, (const float*)c->propg32, (const float*)c->vHS);
-  return propagate_chain<CP_BF16X3, float, float, float>(c, (const float*)c->propg32, (const float*)c->vHS);
+    return propagate_chain<CP_F32, float, float, float, float>(c, c->propg32, c->vHS);
+  return propagate_chain<CP_BF16X3, float, bfs, bfs, bfs>(c, c->propg32, c->vHS);
 }
 
 int do_save_overlaps(afqmc_ctx* c)
@@ -633,7 +648,7 @@ int do_ortho(afqmc_ctx* c)
 
 // Cholesky-form two-body energy from G in [nw][2][N][M] layout (type GT) and
 // Lt [2][K][NM] (type LT_, scaled by sqrt(dt)).
-template <int CP, class GT, class LT_, class YT>
+template <int CP, class GT, class LT_, class YT, class VXT>
 int energy_cholesky(afqmc_ctx* c, const void* Gp)
 {
   const int M = c->M, N = c->N, K = c->K, nw = c->nw;
@@ -655,10 +670,10 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
     using ZT = typename ztype<YT>::type;
     for (int sp = 0; sp < 2; sp++) {
       ZGemmArgs a = gargs();
-      a.A = (const GT*)Gp + 2 * (sp * NM);
+      a.A = zadv<GT>(Gp, sp * NM);
       a.sam = 2 * NM;
       a.sak = 1;
-      a.B = (const YT*)c->Vx + 2 * (sp * NM * NM);
+      a.B = (const typename ztype<VXT>::type*)c->Vx + sp * NM * NM;
       a.sbn = NM;
       a.sbk = 1;
       a.C = c->Ybuf;
@@ -668,7 +683,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
       a.n = (int)NM;
       a.k = (int)NM;
       a.ar = inv_dt;
-      RET((zgemm<CP, GT, YT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+      RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
       RET(launch(c, "energy_exch_dot", [&]() {
         hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream, (int)NM,
                            (const typename ztype<GT>::type*)Gp + sp * NM, 1L, 2 * NM, (const ZT*)c->Ybuf, 1L, NM,
@@ -683,7 +698,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
   for (int n0 = 0; n0 < K; n0 += nq) {
     const int cnt = std::min(nq, K - n0);
     ZGemmArgs a = gargs();
-    a.A = (const LT_*)c->Lt + 2 * ((long)n0 * NM);
+    a.A = zadv<LT_>(c->Lt, (long)n0 * NM);
     a.sam = M;
     a.sak = 1;
     a.sa1 = 0;
@@ -746,7 +761,7 @@ template <class OT> int build_vx(afqmc_ctx* c, const zd* Lt64)
     a.sb1 = 0;
     a.sb2 = M;
     a.bdiv = (int)N;
-    a.C = (OT*)c->Vx + 2 * (sp * NM * NM);  // C_ab(l, k) -> Vx[(a,k)][(b,l)]
+    a.C = (typename ztype<OT>::type*)c->Vx + sp * NM * NM;  // C_ab(l, k) -> Vx[(a,k)][(b,l)]
     a.scm = 1;
     a.scn = NM;
     a.sc1 = M * NM;
@@ -776,7 +791,8 @@ template <class ZT> int maybe_vx(afqmc_ctx* c, const zd* Lt64)
     c->ybuf_elems = nw * NM;
     RET(dalloc_bytes(c, &c->Ybuf, sizeof(ZT) * c->ybuf_elems));
   }
-  if (sizeof(ZT) == sizeof(zd)) return build_vx<double>(c, Lt64);
+  if (std::is_same<ZT, zd>::value) return build_vx<double>(c, Lt64);
+  if (std::is_same<ZT, zb>::value) return build_vx<bfs>(c, Lt64);
   return build_vx<float>(c, Lt64);
 }
 
@@ -806,14 +822,14 @@ int do_energy(afqmc_ctx* c, double* eav_out)
         hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->G, c->Gtmp,
                            (int)NAK, nw, (long)nw, NAK);
       }));
-      RET((energy_cholesky<CP_F64, double, double, double>(c, c->Gtmp)));
+      RET((energy_cholesky<CP_F64, double, double, double, double>(c, c->Gtmp)));
     }
   } else if (c->mode == AFQMC_FP64_DENSE) {
-    RET((energy_cholesky<CP_F64, double, double, double>(c, c->G)));
+    RET((energy_cholesky<CP_F64, double, double, double, double>(c, c->G)));
   } else if (c->mode == AFQMC_MIXED_FP32) {
-    RET((energy_cholesky<CP_F32, float, float, float>(c, c->G)));
+    RET((energy_cholesky<CP_F32, float, float, float, float>(c, c->G)));
   } else {
-    RET((energy_cholesky<CP_BF16X3, float, float, float>(c, c->G)));
+    RET((energy_cholesky<CP_BF16X3, bfs, bfs, float, bfs>(c, c->G)));
   }
   if (eav_out) {
     std::vector<zd> wd((size_t)nw * 8);
@@ -864,6 +880,35 @@ int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt)
 
 bool dense_layout(const afqmc_ctx* c) { return c->mode != AFQMC_FP64_CSR; }
 
+// host split-BF16 packing, same rounding as the device's f2bf
+inline uint16_t host_f2bf(float x)
+{
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+inline float host_bf2f(uint16_t b)
+{
+  uint32_t u = ((uint32_t)b) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+inline void host_split(double re, double im, uint32_t w[2])
+{
+  const float r = (float)re, i = (float)im;
+  const uint16_t hr = host_f2bf(r), hi = host_f2bf(i);
+  const uint16_t lr = host_f2bf(r - host_bf2f(hr)), li = host_f2bf(i - host_bf2f(hi));
+  w[0] = (uint32_t)hr | ((uint32_t)hi << 16);
+  w[1] = (uint32_t)lr | ((uint32_t)li << 16);
+}
+inline zd host_merge(const uint32_t w[2])
+{
+  return zd{(double)host_bf2f((uint16_t)(w[0] & 0xffff)) + (double)host_bf2f((uint16_t)(w[1] & 0xffff)),
+            (double)host_bf2f((uint16_t)(w[0] >> 16)) + (double)host_bf2f((uint16_t)(w[1] >> 16))};
+}
+
 // host transpose helpers: ref [R][C] <-> dev [C][R]
 void htranspose(const zd* in, zd* out, long R, long Cc)
 {
@@ -892,6 +937,7 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   c->nw = nwalk;
   c->mode = mode;
   c->mixed = (mode == AFQMC_MIXED_FP32 || mode == AFQMC_MIXED_BF16);
+  c->bsplit = (mode == AFQMC_MIXED_BF16);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return AFQMC_ERR_HIP;
@@ -1004,9 +1050,14 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
   HIPCHK(c, hipMemcpyAsync(c->propg, propg1, sizeof(zd) * MM, hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->haj, haj, sizeof(zd) * 2 * NM, hipMemcpyHostToDevice, st));
   if (c->mixed) {
-    RET(dalloc(c, &c->propg32, MM));
+    RET(dalloc_bytes(c, &c->propg32, sizeof(zf) * MM));
     RET(launch(c, "convert", [&]() {
-      hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg, c->propg32, MM);
+      if (c->bsplit)
+        hipLaunchKernelGGL((zconvert_kernel<zd, zb>), dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg,
+                           (zb*)c->propg32, MM);
+      else
+        hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg,
+                           (zf*)c->propg32, MM);
     }));
   }
   // Spvn CSR to the device
@@ -1110,7 +1161,8 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     dfree(c, rp);
     dfree(c, col);
     dfree(c, val);
-    if (c->mixed) RET(maybe_vx<zf>(c, Lt64));
+    if (c->bsplit) RET(maybe_vx<zb>(c, Lt64));
+    else if (c->mixed) RET(maybe_vx<zf>(c, Lt64));
     else RET(maybe_vx<zd>(c, Lt64));
     if (c->mixed) {
       zf* Ld32 = nullptr;
@@ -1118,9 +1170,16 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       RET(dalloc(c, &Ld32, K * MM));
       RET(dalloc(c, &Lt32, 2 * K * NM));
       RET(launch(c, "convert", [&]() {
-        hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, Ld32, K * MM);
-        hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64, Lt32,
-                           2 * K * NM);
+        if (c->bsplit) {
+          hipLaunchKernelGGL((zconvert_kernel<zd, zb>), dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, (zb*)Ld32,
+                             K * MM);
+          hipLaunchKernelGGL((zconvert_kernel<zd, zb>), dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64,
+                             (zb*)Lt32, 2 * K * NM);
+        } else {
+          hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, Ld32, K * MM);
+          hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64, Lt32,
+                             2 * K * NM);
+        }
       }));
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Ld64);
@@ -1175,7 +1234,7 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
     HIPCHK(c, hipStreamSynchronize(st));
     return AFQMC_OK;
   };
-  // walker-last reference [R][nw] <-> walker-first device [nw][R] of type zd/zf
+  // walker-last reference [R][nw] <-> walker-first device [nw][R] of type zd/zf/zb
   auto wlast = [&](void* dev, long R, bool is32) -> int {
     std::vector<zd> tmp((size_t)R * nw);
     std::vector<zf> t32;
@@ -1183,7 +1242,15 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
       htranspose((const zd*)host, tmp.data(), R, nw);
       if (is32) {
         t32.resize(tmp.size());
-        for (size_t i = 0; i < tmp.size(); i++) t32[i] = zf{(float)tmp[i].re, (float)tmp[i].im};
+        for (size_t i = 0; i < tmp.size(); i++) {
+          if (c->bsplit) {
+            uint32_t w[2];
+            host_split(tmp[i].re, tmp[i].im, w);
+            std::memcpy(&t32[i], w, sizeof(w));
+          } else {
+            t32[i] = zf{(float)tmp[i].re, (float)tmp[i].im};
+          }
+        }
         HIPCHK(c, hipMemcpyAsync(dev, t32.data(), sizeof(zf) * t32.size(), hipMemcpyHostToDevice, st));
       } else {
         HIPCHK(c, hipMemcpyAsync(dev, tmp.data(), sizeof(zd) * tmp.size(), hipMemcpyHostToDevice, st));
@@ -1194,7 +1261,15 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
         t32.resize(tmp.size());
         HIPCHK(c, hipMemcpyAsync(t32.data(), dev, sizeof(zf) * t32.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
-        for (size_t i = 0; i < tmp.size(); i++) tmp[i] = zd{t32[i].re, t32[i].im};
+        for (size_t i = 0; i < tmp.size(); i++) {
+          if (c->bsplit) {
+            uint32_t w[2];
+            std::memcpy(w, &t32[i], sizeof(w));
+            tmp[i] = host_merge(w);
+          } else {
+            tmp[i] = zd{t32[i].re, t32[i].im};
+          }
+        }
       } else {
         HIPCHK(c, hipMemcpyAsync(tmp.data(), dev, sizeof(zd) * tmp.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));

@@ -67,6 +67,11 @@ template <class Z> __device__ inline void zero_z(Z& z)
   z.re = 0;
   z.im = 0;
 }
+__device__ inline void zero_z(zb& z)
+{
+  z.hi = 0;
+  z.lo = 0;
+}
 
 template <int CP, class AT, class BT, class OT, bool AK, bool BK, int BM, int BNC, int BKC, int WM, int WN>
 __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
@@ -80,6 +85,8 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile must be a multiple of 16");
   static_assert(CP != CP_BF16X3 || BKC % 16 == 0, "bf16 k-tile must be a multiple of 16 complex");
   static_assert(CP == CP_BF16X3 || BKC % 4 == 0, "k-tile must be a multiple of 4 complex");
+  static_assert(CP == CP_BF16X3 || (!std::is_same<AT, bfs>::value && !std::is_same<BT, bfs>::value),
+                "split-BF16 operands need the BF16 path");
   using L = LdsT<CP>;
   using LT = typename L::T;
   constexpr int LDK = BKC + L::PAD;
@@ -111,10 +118,14 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   const long bq = b / p.bdiv, br_ = b % p.bdiv;
   const AZ* __restrict__ A = (const AZ*)p.A + bq * p.sa1 + br_ * p.sa2;
   const BZ* __restrict__ B = (const BZ*)p.B + bq * p.sb1 + br_ * p.sb2;
+  constexpr bool OSPLIT = std::is_same<OT, bfs>::value;
+  using DT = typename std::conditional<OSPLIT, float, OT>::type;  // D/F element
+  using CT = typename std::conditional<OSPLIT, float, OT>::type;  // C element when not split
   const long coff = 2 * (bq * p.sc1 + br_ * p.sc2);
-  OT* C = p.C ? (OT*)p.C + coff : nullptr;
-  OT* D = p.D ? (OT*)p.D + coff : nullptr;
-  const OT* F = p.F ? (const OT*)p.F + coff : nullptr;
+  CT* C = (p.C && !OSPLIT) ? (CT*)p.C + coff : nullptr;
+  zb* CS = (p.C && OSPLIT) ? (zb*)p.C + coff / 2 : nullptr;
+  DT* D = p.D ? (DT*)p.D + coff : nullptr;
+  const DT* F = p.F ? (const DT*)p.F + coff : nullptr;
 
   ACC acc[TM][TN];
 #pragma unroll
@@ -157,12 +168,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         sA[st][0][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
       } else if constexpr (CP == CP_F32) {
         sA[st][0][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
+      } else if constexpr (std::is_same<AZ, zb>::value) {
+        sA[st][0][row * LDK + kc] = ra[i].hi;
+        sA[st][1][row * LDK + kc] = ra[i].lo;
       } else {
-        float xr = (float)ra[i].re, xi = (float)ra[i].im;
-        unsigned short hr = f2bf(xr), hi = f2bf(xi);
-        unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
-        sA[st][0][row * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
-        sA[st][1][row * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+        const zb v = split_z((float)ra[i].re, (float)ra[i].im);
+        sA[st][0][row * LDK + kc] = v.hi;
+        sA[st][1][row * LDK + kc] = v.lo;
       }
     }
 #pragma unroll
@@ -175,12 +187,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         sB[st][0][col * LDK + kc] = zd{(double)rb[i].re, (double)rb[i].im};
       } else if constexpr (CP == CP_F32) {
         sB[st][0][col * LDK + kc] = zf{(float)rb[i].re, (float)rb[i].im};
+      } else if constexpr (std::is_same<BZ, zb>::value) {
+        sB[st][0][col * LDK + kc] = rb[i].hi;
+        sB[st][1][col * LDK + kc] = rb[i].lo;
       } else {
-        float xr = (float)rb[i].re, xi = (float)rb[i].im;
-        unsigned short hr = f2bf(xr), hi = f2bf(xi);
-        unsigned short lr = f2bf(xr - bf2f(hr)), li = f2bf(xi - bf2f(hi));
-        sB[st][0][col * LDK + kc] = (uint32_t)hr | ((uint32_t)hi << 16);
-        sB[st][1][col * LDK + kc] = (uint32_t)lr | ((uint32_t)li << 16);
+        const zb v = split_z((float)rb[i].re, (float)rb[i].im);
+        sB[st][0][col * LDK + kc] = v.hi;
+        sB[st][1][col * LDK + kc] = v.lo;
       }
     }
   };
@@ -283,13 +296,17 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         if (row < p.m && col < p.n) {
           long off = 2 * ((long)row * p.scm + (long)col * p.scn);
           double o = s ? (p.ar * im + p.ai * re) : (p.ar * re - p.ai * im);
-          if (D) D[off + s] = (OT)((F ? (double)F[off + s] : 0.0) + o);
-          if (C) {
+          if (D) D[off + s] = (DT)((F ? (double)F[off + s] : 0.0) + o);
+          if constexpr (OSPLIT) {
+            // lane s=0 writes the whole split complex (alpha applied, no beta)
+            const double oi = __shfl_xor(o, 1);
+            if (CS && s == 0) CS[off / 2] = split_z((float)o, (float)oi);
+          } else if (C) {
             if (has_beta) {
               double cr = (double)C[off], ci = (double)C[off + 1];
               o += s ? (p.br * ci + p.bi * cr) : (p.br * cr - p.bi * ci);
             }
-            C[off + s] = (OT)o;
+            C[off + s] = (CT)o;
           }
         }
       }

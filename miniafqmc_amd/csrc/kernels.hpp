@@ -206,19 +206,14 @@ __global__ __launch_bounds__(256) void ztranspose_kernel(const IT* __restrict__ 
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int yy = ty; yy < 32; yy += 8) {
     int rr = by + yy, cc = bx + tx;
-    if (rr < R && cc < Cc) {
-      IT v = in[(long)rr * ldi + cc];
-      tile[yy][tx] = zd{(double)v.re, (double)v.im};
-    }
+    if (rr < R && cc < Cc) tile[yy][tx] = to_zd(in[(long)rr * ldi + cc]);
   }
   __syncthreads();
   for (int yy = ty; yy < 32; yy += 8) {
     int cc = bx + yy, rr = by + tx;
     if (rr < R && cc < Cc) {
-      zd v = tile[tx][yy];
       OT o;
-      o.re = v.re;
-      o.im = v.im;
+      from_zd(tile[tx][yy], o);
       out[(long)cc * ldo + rr] = o;
     }
   }
@@ -231,10 +226,8 @@ __global__ void zconvert_kernel(const IT* __restrict__ in, OT* __restrict__ out,
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long st = (long)gridDim.x * blockDim.x;
   for (; i < n; i += st) {
-    IT v = in[i];
     OT o;
-    o.re = v.re;
-    o.im = v.im;
+    from_zd(to_zd(in[i]), o);
     out[i] = o;
   }
 }
@@ -264,8 +257,7 @@ __global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __
   bool bad = false;
   if (w < nw) {
     for (int n = wv; n < K; n += 4) {
-      VT v = vb[(long)n * vsn + (long)w * vsw];
-      zd b{(double)v.re, (double)v.im};
+      const zd b = to_zd(vb[(long)n * vsn + (long)w * vsw]);
       double xr = x[(long)n * nw + w];
       if (!isfinite(xr)) bad = true;
       // t1 = i*vb ; t3 = x + i/2*vb ; hybridW -= t1*t3
@@ -275,8 +267,7 @@ __global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __
       h.re -= t4.re;
       h.im -= t4.im;
       XT o;
-      o.re = xr - b.im;
-      o.im = b.re;
+      from_zd(zd{xr - b.im, b.re}, o);
       X[(long)n * xsn + (long)w * xsw] = o;
     }
   }
@@ -666,10 +657,10 @@ __global__ __launch_bounds__(256) void coldot_kernel(int n, const AT* __restrict
   const int w = blockIdx.x;
   zd acc{0, 0};
   for (int i = threadIdx.x; i < n; i += 256) {
-    AT a = A[(long)i * asi + (long)w * asw];
-    BT bb = B[(long)i * bsi + (long)w * bsw];
-    acc.re += (double)a.re * (double)bb.re - (double)a.im * (double)bb.im;
-    acc.im += (double)a.re * (double)bb.im + (double)a.im * (double)bb.re;
+    const zd a = to_zd(A[(long)i * asi + (long)w * asw]);
+    const zd bb = to_zd(B[(long)i * bsi + (long)w * bsw]);
+    acc.re += a.re * bb.re - a.im * bb.im;
+    acc.im += a.re * bb.im + a.im * bb.re;
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -699,10 +690,10 @@ __global__ __launch_bounds__(256) void trace_sq_kernel(const YT* __restrict__ Y,
   for (long e = threadIdx.x; e < (long)nq * NN; e += 256) {
     long n = e / NN, rem = e % NN;
     int a = (int)(rem / N), b = (int)(rem % N);
-    YT y1 = Yq[n * NN + (long)a * N + b];
-    YT y2 = Yq[n * NN + (long)b * N + a];
-    acc.re += (double)y1.re * (double)y2.re - (double)y1.im * (double)y2.im;
-    acc.im += (double)y1.re * (double)y2.im + (double)y1.im * (double)y2.re;
+    const zd y1 = to_zd(Yq[n * NN + (long)a * N + b]);
+    const zd y2 = to_zd(Yq[n * NN + (long)b * N + a]);
+    acc.re += y1.re * y2.re - y1.im * y2.im;
+    acc.im += y1.re * y2.im + y1.im * y2.re;
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
@@ -729,9 +720,9 @@ __global__ __launch_bounds__(256) void sumsq_kernel(int K, const VT* __restrict_
   const int w = blockIdx.x;
   zd acc{0, 0};
   for (int n = threadIdx.x; n < K; n += 256) {
-    VT x = v[(long)n * vsn + (long)w * vsw];
-    acc.re += (double)x.re * (double)x.re - (double)x.im * (double)x.im;
-    acc.im += 2.0 * (double)x.re * (double)x.im;
+    const zd x = to_zd(v[(long)n * vsn + (long)w * vsw]);
+    acc.re += x.re * x.re - x.im * x.im;
+    acc.im += 2.0 * x.re * x.im;
   }
   sh[threadIdx.x] = acc;
   __syncthreads();
