@@ -54,7 +54,7 @@ def kernel_flops(name, M, N, K, nw):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="N128_M512")
     ap.add_argument("--mode", default="mixed_bf16")
@@ -62,7 +62,7 @@ def main():
     ap.add_argument("--walkers", type=int, default=None)
     ap.add_argument("--fp64", type=int, default=1, help="also run the FP64 GPU path (0 to skip)")
     ap.add_argument("--cpu", type=int, default=1, help="time the oracle on a walker sample (0 to skip)")
-    ap.add_argument("--cpu-walkers", type=int, default=1)
+    ap.add_argument("--cpu-walkers", type=int, default=6)
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     args = ap.parse_args()
@@ -84,7 +84,11 @@ def main():
     x = m.fields_block(p["seed"], K, nw, 0, nsub)
     print(f"[bench] system M={M} N={N} K={K} nnz={s.spvn.nnz} walkers={nw} generated in {t_gen:.1f}s", flush=True)
 
-    def run_mode(mode, profile):
+    def run_mode(mode, profile, eval_on=None):
+        """Time `args.steps` substeps in `mode`.  With eval_on=(W, W_data) the
+        local energy of those walkers is evaluated afterwards in this mode
+        (untimed), for the per-walker error of one mode's energy against
+        another's on identical walkers."""
         g = m.AfqmcSys(s, nw, mode)
         g.stage_fields(x)
         if args.tune:
@@ -107,11 +111,23 @@ def main():
         stats = g.kernel_stats() if profile else {}
         if profile:
             g.profile(False)
-        Wd = g.W_data
+        out = {"Wd": g.W_data, "stats": stats}
+        if not args.no_energy:
+            out["terms"] = g.energy_terms()
+            if eval_on is None and args.fp64:
+                out["W"] = g.W
+        if eval_on is not None:
+            g.set(m._native.BUF_W, eval_on[0])
+            g.set(m._native.BUF_WDATA, eval_on[1])
+            g.calculate_mixed_density_matrix()
+            g.calculate_energy()
+            out["eval_Wd"] = g.W_data
+            out["eval_terms"] = g.energy_terms()
         g.close()
-        return tb - ta, eav, Wd, stats
+        return tb - ta, eav, out
 
-    T, eav, Wd, stats = run_mode(args.mode, True)
+    T, eav, res = run_mode(args.mode, True)
+    Wd, stats = res["Wd"], res["stats"]
     value = nw * args.steps / T
     print(f"[bench] {args.mode}: {T:.3f}s for {args.steps} substeps -> {value:.1f} walker-steps/s", flush=True)
     for k_, (n_, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
@@ -132,18 +148,32 @@ def main():
 
     extra = {}
     if args.fp64 and args.mode != "fp64_dense":
-        T64, eav64, Wd64, _ = run_mode("fp64_dense", False)
+        ev = None if args.no_energy else (res.pop("W"), Wd)
+        T64, eav64, r64 = run_mode("fp64_dense", False, eval_on=ev)
+        Wd64 = r64["Wd"]
         v64 = nw * args.steps / T64
-        dE = np.abs(Wd[:, 0] - Wd64[:, 0])
         extra["fp64_gpu"] = {"value": round(v64, 2), "ms_per_step": round(1e3 * T64 / args.steps, 3)}
         extra["speedup_vs_fp64"] = round(value / v64, 3)
         if not args.no_energy:
-            extra["max_abs_dE_L_Ha"] = float(dE.max())
-            extra["mean_abs_dE_L_Ha"] = float(dE.mean())
-            extra["dEav_Ha"] = float(abs(eav[-1] - eav64[-1]))
+            # (1) energy evaluation: both precisions on the same walkers (the
+            # mixed run's final ones); (2) trajectory: each run's own walkers
+            # after the same fields
+            dEe = np.abs(Wd[:, 0] - r64["eval_Wd"][:, 0])
+            dTe = np.abs(res["terms"] - r64["eval_terms"]).max(axis=0)
+            dE = np.abs(Wd[:, 0] - Wd64[:, 0])
+            extra["energy_eval_error"] = {
+                "max_abs_dE_L_Ha": float(dEe.max()), "mean_abs_dE_L_Ha": float(dEe.mean()),
+                "max_abs_dterm_Ha": {"one_body": float(dTe[0]), "coulomb": float(dTe[1]), "exchange": float(dTe[2])},
+                "E_L_terms_mean_Ha": [float(v) for v in r64["eval_terms"].real.mean(axis=0)],
+                "target_Ha": 1e-5,
+            }
+            extra["trajectory_error"] = {
+                "max_abs_dE_L_Ha": float(dE.max()), "mean_abs_dE_L_Ha": float(dE.mean()),
+                "dEav_Ha": float(abs(eav[-1] - eav64[-1])), "substeps": args.warmup + args.steps,
+            }
             extra["E_L_mean_Ha"] = float(Wd64[:, 0].real.mean())
         print(f"[bench] fp64_dense: {v64:.1f} walker-steps/s; mixed/fp64 = {value / v64:.2f}x; "
-              f"max|dE_L| = {extra.get('max_abs_dE_L_Ha')}", flush=True)
+              f"energy eval {extra.get('energy_eval_error')}; trajectory {extra.get('trajectory_error')}", flush=True)
 
     cpu = None
     if args.cpu:

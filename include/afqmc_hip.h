@@ -63,7 +63,12 @@ typedef enum {
   AFQMC_BUF_VBIAS = 3,   /* force bias     [K][nwalk] */
   AFQMC_BUF_X = 4,       /* shifted fields [K][nwalk] */
   AFQMC_BUF_VHS = 5,     /* HS potential   [M*M][nwalk] */
-  AFQMC_BUF_HYBRIDW = 6  /* hybrid weight  [nwalk] */
+  AFQMC_BUF_HYBRIDW = 6, /* hybrid weight  [nwalk] */
+  /* terms of the last local energy, [nwalk][3] = (one-body, Coulomb,
+   * exchange), E_L = their sum; read-only.  With Vakbl the whole two-body
+   * energy sum G.(Vakbl G)/2 (energy.hpp:93-95) is the second term and the
+   * third is zero. */
+  AFQMC_BUF_EPARTS = 7
 } afqmc_buffer;
 
 /* ---- context ------------------------------------------------------------ */
@@ -81,7 +86,13 @@ typedef enum {
   /* 1 (default): keep the same-spin exchange block of Vakbl dense in device
    * memory, [2][NM][NM] in the contraction precision, when it fits, so the
    * two-body energy is one GEMM per spin; 0: always use the Cholesky form. */
-  AFQMC_OPT_DENSE_EXCHANGE = 0
+  AFQMC_OPT_DENSE_EXCHANGE = 0,
+  /* mixed modes only.  1 (default): the local energy is evaluated from an
+   * FP64 density matrix (rebuilt from the FP64 inverses), with the one-body
+   * and Coulomb terms in FP64 (an FP64 copy of the half-rotated vectors,
+   * 2*K*N*M complex doubles, is kept); the exchange term stays in the
+   * contraction precision.  0: the whole energy in the contraction precision. */
+  AFQMC_OPT_ENERGY_FP64 = 1
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

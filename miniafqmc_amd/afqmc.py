@@ -23,14 +23,15 @@ def _dp(a):
 
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
-                 use_vakbl: bool = True, dense_exchange: bool = True):
+                 use_vakbl: bool = True, dense_exchange: bool = True, energy_fp64: bool = True):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
         h = C.c_void_p()
         nat.check(nat.lib.afqmc_ctx_create(self.M, self.N, self.K, nwalk, nat.MODES[mode], C.byref(h)))
         self.ctx = h
-        self._check(nat.lib.afqmc_set_option(self.ctx, 0, 1.0 if dense_exchange else 0.0))
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DENSE_EXCHANGE, 1.0 if dense_exchange else 0.0))
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ENERGY_FP64, 1.0 if energy_fp64 else 0.0))
         ta = np.ascontiguousarray(system.trial_alpha)
         tb = np.ascontiguousarray(system.trial_beta)
         sp = system.spvn.view()
@@ -66,6 +67,7 @@ class AfqmcSys:
         nat.BUF_X: lambda s: (s.K, s.nwalk),
         nat.BUF_VHS: lambda s: (s.M * s.M, s.nwalk),
         nat.BUF_HYBRIDW: lambda s: (s.nwalk,),
+        nat.BUF_EPARTS: lambda s: (s.nwalk, 3),
     }
 
     def get(self, buf):
@@ -139,6 +141,10 @@ class AfqmcSys:
         e = C.c_double(0.0)
         self._check(nat.lib.afqmc_energy(self.ctx, C.byref(e)))
         return e.value
+
+    def energy_terms(self):
+        """[nwalk][3] one-body, Coulomb and exchange terms of the last local energy."""
+        return self.get(nat.BUF_EPARTS)
 
     def spvnT(self):
         """Half-rotated Cholesky matrix built at upload (rotate.hpp:65-169)."""

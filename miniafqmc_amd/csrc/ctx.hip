@@ -68,6 +68,8 @@ struct afqmc_ctx {
   long long v_nnz = 0;
   void* Ld = nullptr;  // dense modes: [K][M*M]  (zd or zf)
   void* Lt = nullptr;  // [2][K][N*M] (zd or zf); scaled by sqrt(dt)
+  zd* Lt64 = nullptr;  // mixed modes with the FP64 energy: Lt in FP64
+  bool energy_fp64 = true;  // mixed modes: density matrix, one-body and Coulomb energy in FP64
 
   // walker state and per-substep buffers
   zd* W = nullptr;
@@ -75,6 +77,7 @@ struct afqmc_ctx {
   zd* T1 = nullptr;       // [nw][2][N][N] overlap matrices, then their inverses
   zd* dcache = nullptr;   // [nw][2] determinants of the last inversion
   bool t1inv_valid = false;  // T1 holds the inverses for the current walkers
+  bool g_fresh = false;      // G was built from T1 and W as they are now
   void* G = nullptr;      // CSR: zd [2NM][nw]; FP64_DENSE: zd [nw][2NM]; mixed: zf [nw][2NM]
   zd* vbias = nullptr;    // CSR: [K][nw]; dense: [nw][K]
   void* X = nullptr;      // CSR: zd [K][nw]; dense: [nw][K] (zd / zf)
@@ -101,7 +104,7 @@ struct afqmc_ctx {
   // the contraction precision; null when it does not fit in device memory
   void* Vx = nullptr;
   bool allow_vx = true;
-  zd* Gtmp = nullptr;     // CSR mode Cholesky energy: G transposed to [nw][2NM]
+  zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
   zd* Gcloc = nullptr;    // CSR mode Vakbl energy
 
   // profiling
@@ -383,22 +386,11 @@ int overlap_matrices(afqmc_ctx* c)
   return zgemm<CP_F64, double, double, double, false, false>(c, "ovlp_gemm", a, 2 * c->nw);
 }
 
-// ---------------------------------------------------------------------------
-int do_mdm(afqmc_ctx* c)
+// Gc(a,k) = sum_b T1inv(a,b) W(k,b) for every walker-spin pair; output
+// strides are set by the caller.
+ZGemmArgs gc_args(const afqmc_ctx* c)
 {
-  const int M = c->M, N = c->N, nw = c->nw;
-  if (c->t1inv_valid) {
-    // the overlap step already inverted W^T conj(Psi) for these walkers
-    RET(launch(c, "det_restore", [&]() {
-      hipLaunchKernelGGL(det_restore_kernel, dim3((2 * nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->dcache,
-                         nw);
-    }));
-  } else {
-    RET(overlap_matrices(c));
-    RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
-    c->t1inv_valid = true;
-  }
-  // Gc(a,k) = sum_b T1inv(a,b) W(k,b)
+  const int M = c->M, N = c->N;
   ZGemmArgs a = gargs();
   a.A = c->T1;
   a.sam = N;
@@ -414,6 +406,51 @@ int do_mdm(afqmc_ctx* c)
   a.m = N;
   a.n = M;
   a.k = N;
+  return a;
+}
+
+// FP64 copy of G in [nw][2NM] for the energy of the mixed modes: rebuilt
+// from the inverses when G is current for the walkers, otherwise widened
+// from the stored G (a G uploaded by the caller, or a stale one).
+int g_fp64(afqmc_ctx* c)
+{
+  const long NAK = 2L * c->N * c->M, nw = c->nw;
+  if (c->g_fresh && c->t1inv_valid) {
+    ZGemmArgs a = gc_args(c);
+    a.C = c->Gtmp;
+    a.scm = c->M;
+    a.scn = 1;
+    a.sc1 = NAK;
+    a.sc2 = NAK / 2;
+    return zgemm<CP_F64, double, double, double, true, true>(c, "gc64_gemm", a, 2 * c->nw);
+  }
+  return launch(c, "convert", [&]() {
+    if (c->bsplit)
+      hipLaunchKernelGGL((zconvert_kernel<zb, zd>), dim3(nblk(NAK * nw, 256)), dim3(256), 0, c->stream,
+                         (const zb*)c->G, c->Gtmp, NAK * nw);
+    else
+      hipLaunchKernelGGL((zconvert_kernel<zf, zd>), dim3(nblk(NAK * nw, 256)), dim3(256), 0, c->stream,
+                         (const zf*)c->G, c->Gtmp, NAK * nw);
+  });
+}
+
+// ---------------------------------------------------------------------------
+int do_mdm(afqmc_ctx* c)
+{
+  const int M = c->M, N = c->N, nw = c->nw;
+  if (c->t1inv_valid) {
+    // the overlap step already inverted W^T conj(Psi) for these walkers
+    RET(launch(c, "det_restore", [&]() {
+      hipLaunchKernelGGL(det_restore_kernel, dim3((2 * nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->dcache,
+                         nw);
+    }));
+  } else {
+    RET(overlap_matrices(c));
+    RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
+    c->t1inv_valid = true;
+  }
+  c->g_fresh = true;
+  ZGemmArgs a = gc_args(c);
   a.C = c->G;
   if (c->mode == AFQMC_FP64_CSR) {
     // G[(s*N + a)*M + k][w]
@@ -436,6 +473,7 @@ int do_mdm(afqmc_ctx* c)
 // matrix of the same walkers.
 int do_overlaps(afqmc_ctx* c)
 {
+  c->g_fresh = false;
   RET(overlap_matrices(c));
   RET(gj_inverse(c, c->T1, c->T1, c->Wd + 2, 8, 1));
   c->t1inv_valid = true;
@@ -444,7 +482,7 @@ int do_overlaps(afqmc_ctx* c)
 
 // vbias(w, n) = sum_{s,j} G(w, s, j) Lt(s, n, j) in dense modes.
 template <int CP, class GT, class LT_>
-int dense_contract_vbias(afqmc_ctx* c, const void* Gp, zd* out, const char* name)
+int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, const char* name)
 {
   const long NM = (long)c->N * c->M;
   for (int sp = 0; sp < 2; sp++) {
@@ -452,7 +490,7 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, zd* out, const char* name
     a.A = zadv<GT>(Gp, sp * NM);
     a.sam = 2 * NM;
     a.sak = 1;
-    a.B = zadv<LT_>(c->Lt, sp * (long)c->K * NM);
+    a.B = zadv<LT_>(Lt, sp * (long)c->K * NM);
     a.sbn = NM;
     a.sbk = 1;
     a.C = out;
@@ -478,9 +516,11 @@ int do_vbias(afqmc_ctx* c)
                          c->spt_col, c->spt_val, (const zd*)c->G, (long)nw, c->vbias, (long)nw, one, zero);
     });
   }
-  if (c->mode == AFQMC_FP64_DENSE) return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->vbias, "vbias_gemm");
-  if (c->mode == AFQMC_MIXED_FP32) return dense_contract_vbias<CP_F32, float, float>(c, c->G, c->vbias, "vbias_gemm");
-  return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->vbias, "vbias_gemm");
+  if (c->mode == AFQMC_FP64_DENSE)
+    return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
+  if (c->mode == AFQMC_MIXED_FP32)
+    return dense_contract_vbias<CP_F32, float, float>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
+  return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
 }
 
 int do_fields(afqmc_ctx* c, const double* xdev)
@@ -609,6 +649,7 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
 int do_propagate(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
+  c->g_fresh = false;
   if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHSw);
   if (c->mode == AFQMC_FP64_DENSE)
     return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHS);
@@ -640,16 +681,19 @@ size_t lq_lds_bytes(int M) { return sizeof(zd) * ((size_t)kLqNB * M + 128 * kLqN
 int do_ortho(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
+  c->g_fresh = false;
   const size_t lds = lq_lds_bytes(c->M);
   return launch(c, "lq_orthogonalize", [&]() {
     hipLaunchKernelGGL(lq_blocked_kernel<kLqNB>, dim3(2 * c->nw), dim3(256), lds, c->stream, c->W, c->M, c->N);
   });
 }
 
-// Cholesky-form two-body energy from G in [nw][2][N][M] layout (type GT) and
-// Lt [2][K][NM] (type LT_, scaled by sqrt(dt)).
-template <int CP, class GT, class LT_, class YT, class VXT>
-int energy_cholesky(afqmc_ctx* c, const void* Gp)
+// Local energy from G in [nw][2][N][M] layout (type GT): one-body haj.G,
+// Coulomb term from v = Lv.G (Lv [2][K][NM] of type LVT, scaled by sqrt(dt),
+// contracted in precision CPV) and exchange from the dense block Vx or the
+// Cholesky form with Lt (type LT_), both in precision CP.
+template <int CPV, class LVT, int CP, class GT, class LT_, class YT, class VXT>
+int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv)
 {
   const int M = c->M, N = c->N, K = c->K, nw = c->nw;
   const long NM = (long)N * M;
@@ -659,7 +703,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp)
     (void)hipMemsetAsync(c->e_exch_part, 0, sizeof(zd) * 2 * nw, c->stream);
   }));
   // direct term: v(w,n) = sum Lt G ; direct = sum_n v^2 / dt
-  RET((dense_contract_vbias<CP, GT, LT_>(c, Gp, c->e_v, "energy_v_gemm")));
+  RET((dense_contract_vbias<CPV, GT, LVT>(c, Gp, Lv, c->e_v, "energy_v_gemm")));
   RET(launch(c, "energy_direct", [&]() {
     hipLaunchKernelGGL((sumsq_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, c->e_v, 1L, (long)K, c->e_direct,
                        inv_dt);
@@ -808,13 +852,20 @@ int do_energy(afqmc_ctx* c, double* eav_out)
         hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, (int)NAK, nw, (int)NAK,
                            c->v_rp, c->v_col, c->v_val, (const zd*)c->G, (long)nw, c->Gcloc, (long)nw, one, zero);
       }));
+      // E2 = sum G.(Vakbl G) lands in the direct slot (exchange slot zero),
+      // so E_L = E2/2 + E1 as energy.hpp:93-100
       RET(launch(c, "energy_two", [&]() {
+        (void)hipMemsetAsync(c->e_exch, 0, sizeof(zd) * nw, c->stream);
         hipLaunchKernelGGL((coldot_kernel<zd, zd>), dim3(nw), dim3(256), 0, c->stream, (int)NAK, (const zd*)c->G,
-                           (long)nw, 1L, (const zd*)c->Gcloc, (long)nw, 1L, c->Wd, 8L, 0.5, 0);
+                           (long)nw, 1L, (const zd*)c->Gcloc, (long)nw, 1L, c->e_direct, 1L, 1.0, 0);
       }));
       RET(launch(c, "energy_one", [&]() {
         hipLaunchKernelGGL((coldot_kernel<zd, zd>), dim3(nw), dim3(256), 0, c->stream, (int)NAK, (const zd*)c->G,
-                           (long)nw, 1L, (const zd*)c->haj, 1L, 0L, c->Wd, 8L, 1.0, 1);
+                           (long)nw, 1L, (const zd*)c->haj, 1L, 0L, c->e_one, 1L, 1.0, 0);
+      }));
+      RET(launch(c, "energy_assemble", [&]() {
+        hipLaunchKernelGGL(energy_assemble_kernel, dim3((nw + 255) / 256), dim3(256), 0, c->stream, c->Wd,
+                           c->e_direct, c->e_exch, c->e_one, nw);
       }));
     } else {
       dim3 grid((nw + 31) / 32, (unsigned)((NAK + 31) / 32));
@@ -822,14 +873,20 @@ int do_energy(afqmc_ctx* c, double* eav_out)
         hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->G, c->Gtmp,
                            (int)NAK, nw, (long)nw, NAK);
       }));
-      RET((energy_cholesky<CP_F64, double, double, double, double>(c, c->Gtmp)));
+      RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->Gtmp, c->Lt)));
     }
   } else if (c->mode == AFQMC_FP64_DENSE) {
-    RET((energy_cholesky<CP_F64, double, double, double, double>(c, c->G)));
+    RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->G, c->Lt)));
+  } else if (c->Lt64) {
+    // FP64 density matrix, one-body and Coulomb terms; exchange in the
+    // contraction precision
+    RET(g_fp64(c));
+    if (c->bsplit) RET((energy_cholesky<CP_F64, double, CP_BF16X3, double, bfs, float, bfs>(c, c->Gtmp, c->Lt64)));
+    else RET((energy_cholesky<CP_F64, double, CP_F32, double, float, float, float>(c, c->Gtmp, c->Lt64)));
   } else if (c->mode == AFQMC_MIXED_FP32) {
-    RET((energy_cholesky<CP_F32, float, float, float, float>(c, c->G)));
+    RET((energy_cholesky<CP_F32, float, CP_F32, float, float, float, float>(c, c->G, c->Lt)));
   } else {
-    RET((energy_cholesky<CP_BF16X3, bfs, bfs, float, bfs>(c, c->G)));
+    RET((energy_cholesky<CP_BF16X3, bfs, CP_BF16X3, bfs, bfs, float, bfs>(c, c->G, c->Lt)));
   }
   if (eav_out) {
     std::vector<zd> wd((size_t)nw * 8);
@@ -1018,9 +1075,10 @@ int afqmc_sync(afqmc_ctx* c)
 int afqmc_set_option(afqmc_ctx* c, int option, double value)
 {
   if (!c) return AFQMC_ERR_ARG;
-  if (option == AFQMC_OPT_DENSE_EXCHANGE) {
+  if (option == AFQMC_OPT_DENSE_EXCHANGE || option == AFQMC_OPT_ENERGY_FP64) {
     if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
-    c->allow_vx = value != 0.0;
+    if (option == AFQMC_OPT_DENSE_EXCHANGE) c->allow_vx = value != 0.0;
+    else c->energy_fp64 = value != 0.0;
     return AFQMC_OK;
   }
   return fail(c, AFQMC_ERR_ARG, "unknown option");
@@ -1183,7 +1241,12 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       }));
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Ld64);
-      dfree(c, Lt64);
+      if (c->energy_fp64) {
+        c->Lt64 = Lt64;
+        RET(dalloc(c, &c->Gtmp, 2 * NM * c->nw));
+      } else {
+        dfree(c, Lt64);
+      }
       c->Ld = Ld32;
       c->Lt = Lt32;
     } else {
@@ -1215,6 +1278,7 @@ int afqmc_init_walkers(afqmc_ctx* c)
   if (!c->have_system) return fail(c, AFQMC_ERR_STATE, "upload a system first");
   const long MN = (long)c->M * c->N;
   c->t1inv_valid = false;
+  c->g_fresh = false;
   long tot = std::max<long>((long)c->nw * 2 * MN, (long)c->nw * 8);
   return launch(c, "init_walkers", [&]() {
     hipLaunchKernelGGL(init_walkers_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->W, c->Wd,
@@ -1280,6 +1344,7 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
   };
   const bool d = dense_layout(c);
   if (to_device && buffer == AFQMC_BUF_W) c->t1inv_valid = false;
+  if (to_device && (buffer == AFQMC_BUF_W || buffer == AFQMC_BUF_G)) c->g_fresh = false;
   switch (buffer) {
   case AFQMC_BUF_W: return raw(c->W, sizeof(zd) * nw * 2 * MN);
   case AFQMC_BUF_WDATA: return raw(c->Wd, sizeof(zd) * nw * 8);
@@ -1298,6 +1363,21 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
       }));
     }
     return AFQMC_OK;
+  case AFQMC_BUF_EPARTS: {
+    if (to_device) return fail(c, AFQMC_ERR_ARG, "the energy terms are read-only");
+    std::vector<zd> e1(nw), ed(nw), ex(nw);
+    HIPCHK(c, hipMemcpyAsync(e1.data(), c->e_one, sizeof(zd) * nw, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(ed.data(), c->e_direct, sizeof(zd) * nw, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(ex.data(), c->e_exch, sizeof(zd) * nw, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    zd* o = (zd*)host;
+    for (long w = 0; w < nw; w++) {
+      o[3 * w] = e1[w];
+      o[3 * w + 1] = zd{0.5 * ed[w].re, 0.5 * ed[w].im};
+      o[3 * w + 2] = zd{-0.5 * ex[w].re, -0.5 * ex[w].im};
+    }
+    return AFQMC_OK;
+  }
   default: return fail(c, AFQMC_ERR_ARG, "unknown buffer");
   }
 }
@@ -1454,6 +1534,7 @@ int afqmc_tune(afqmc_ctx* c, double dt)
   if (r == AFQMC_OK) r = do_energy(c, nullptr);
   c->tuning = false;
   c->t1inv_valid = false;
+  c->g_fresh = false;
   if (std::getenv("AFQMC_VERBOSE"))
     for (auto& kv : c->tune) std::fprintf(stderr, "[afqmc] tile %-48s -> candidate %d\n", kv.first.c_str(), kv.second);
   HIPCHK(c, hipStreamSynchronize(c->stream));

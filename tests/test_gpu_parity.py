@@ -229,3 +229,52 @@ def test_run_mixed_error_vs_fp64_oracle(c1, mode, tol):
     dEav = np.abs(g["eav"] - o["eav"])
     say(f"{mode}: max |dE_L| = {dE.max():.3e} Ha, max |dEav| = {dEav.max():.3e} Ha")
     assert dE.max() < tol and dEav.max() < tol
+
+
+def test_energy_terms_sum_and_one_body(c1):
+    """E_L splits into one-body + Coulomb + exchange; the one-body term is haj.G (energy.hpp:96-100)."""
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=11)
+    for mode in ("fp64_csr", "fp64_dense"):
+        g = m.AfqmcSys(s, nw, mode)
+        g.set(nat.BUF_W, W)
+        wd = np.zeros((nw, 8), complex)
+        wd[:, 1] = 1
+        g.set(nat.BUF_WDATA, wd)
+        G = g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        t = g.energy_terms()
+        close(t.sum(axis=1), g.W_data[:, 0], 1e-13, f"{mode}: sum of terms vs E_L")
+        e1 = np.einsum("iw,i->w", G, s.haj.reshape(-1))
+        close(t[:, 0], e1, 1e-12, f"{mode}: one-body term")
+        g.close()
+
+
+@pytest.mark.parametrize("mode", ["mixed_fp32", "mixed_bf16"])
+def test_mixed_energy_evaluation_error(c1, mode):
+    """Per-walker E_L of the mixed modes on identical walkers vs the FP64 oracle.
+
+    Default (FP64 density matrix, one-body and Coulomb; exchange in the
+    contraction precision): |dE_L| < 1e-5 Ha.  Whole energy in the contraction
+    precision: reported, bounded at 1e-3 Ha.
+    """
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=13)
+    wd = np.zeros((nw, 8), complex)
+    wd[:, 1] = 1
+    Wdo = wd.copy()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wdo)
+    orc.calculate_energy(s, Wdo, Go)
+    eo = Wdo[:, 0]
+    for fp64, tol in ((True, 1e-5), (False, 1e-3)):
+        g = m.AfqmcSys(s, nw, mode, energy_fp64=fp64)
+        g.set(nat.BUF_W, W)
+        g.set(nat.BUF_WDATA, wd)
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        dE = np.abs(g.W_data[:, 0] - eo)
+        say(f"{mode} energy_fp64={fp64}: max |dE_L| = {dE.max():.3e} Ha (E_L ~ {eo.real.mean():.3f} Ha)")
+        assert dE.max() < tol
+        g.close()
