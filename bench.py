@@ -73,6 +73,7 @@ def main():
         sys.exit(2)
 
     import miniafqmc_amd as m
+    from miniafqmc_amd import _native as nat
 
     t0 = time.time()
     s, p = m.preset_system(args.workload)
@@ -84,11 +85,33 @@ def main():
     x = m.fields_block(p["seed"], K, nw, 0, nsub)
     print(f"[bench] system M={M} N={N} K={K} nnz={s.spvn.nnz} walkers={nw} generated in {t_gen:.1f}s", flush=True)
 
-    def run_mode(mode, profile, eval_on=None):
-        """Time `args.steps` substeps in `mode`.  With eval_on=(W, W_data) the
-        local energy of those walkers is evaluated afterwards in this mode
-        (untimed), for the per-walker error of one mode's energy against
-        another's on identical walkers."""
+    probe_steps = (1, 10)
+
+    def probe(g, state):
+        """Untimed accuracy probes from a given walker state (W, W_data, Eshift):
+        the local energy of those walkers, then of the walkers after k substeps
+        with the same fields, for k in probe_steps."""
+        W0, Wd0, es0 = state
+        out = {}
+        g.set(nat.BUF_W, W0)
+        g.set(nat.BUF_WDATA, Wd0)
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        out["eval"] = (g.W_data[:, 0], g.energy_terms())
+        for k in probe_steps:
+            g.set(nat.BUF_W, W0)
+            g.set(nat.BUF_WDATA, Wd0)
+            g.set_eshift(es0)
+            for t in range(k):
+                g.substep(t, t > 0 and t % northo == 0, s.dt)
+            g.calculate_mixed_density_matrix()
+            g.calculate_energy()
+            out[k] = g.W_data[:, 0]
+        return out
+
+    def run_mode(mode, profile, state=None):
+        """Time `args.steps` substeps in `mode`; then run the accuracy probes
+        from `state`, or from this run's own final walkers (returned)."""
         g = m.AfqmcSys(s, nw, mode)
         g.stage_fields(x)
         if args.tune:
@@ -112,17 +135,11 @@ def main():
         if profile:
             g.profile(False)
         out = {"Wd": g.W_data, "stats": stats}
-        if not args.no_energy:
-            out["terms"] = g.energy_terms()
-            if eval_on is None and args.fp64:
-                out["W"] = g.W
-        if eval_on is not None:
-            g.set(m._native.BUF_W, eval_on[0])
-            g.set(m._native.BUF_WDATA, eval_on[1])
-            g.calculate_mixed_density_matrix()
-            g.calculate_energy()
-            out["eval_Wd"] = g.W_data
-            out["eval_terms"] = g.energy_terms()
+        if not args.no_energy and args.fp64:
+            if state is None:
+                state = (g.W, out["Wd"], g.eshift())
+                out["state"] = state
+            out["probe"] = probe(g, state)
         g.close()
         return tb - ta, eav, out
 
@@ -148,32 +165,43 @@ def main():
 
     extra = {}
     if args.fp64 and args.mode != "fp64_dense":
-        ev = None if args.no_energy else (res.pop("W"), Wd)
-        T64, eav64, r64 = run_mode("fp64_dense", False, eval_on=ev)
+        T64, eav64, r64 = run_mode("fp64_dense", True, state=res.get("state"))
         Wd64 = r64["Wd"]
         v64 = nw * args.steps / T64
-        extra["fp64_gpu"] = {"value": round(v64, 2), "ms_per_step": round(1e3 * T64 / args.steps, 3)}
+        extra["fp64_gpu"] = {"value": round(v64, 2), "ms_per_step": round(1e3 * T64 / args.steps, 3),
+                             "kernels_ms": {k_: round(ms, 2) for k_, (n_, ms) in r64["stats"].items() if ms > 1.0}}
         extra["speedup_vs_fp64"] = round(value / v64, 3)
         if not args.no_energy:
-            # (1) energy evaluation: both precisions on the same walkers (the
-            # mixed run's final ones); (2) trajectory: each run's own walkers
-            # after the same fields
-            dEe = np.abs(Wd[:, 0] - r64["eval_Wd"][:, 0])
-            dTe = np.abs(res["terms"] - r64["eval_terms"]).max(axis=0)
+            pm, p64 = res["probe"], r64["probe"]
+            e64 = p64["eval"][0]
+            Eref = float(np.abs(e64).mean())
+
+            def err(a, b):
+                d = np.abs(a - b)
+                return {"max_abs_dE_L_Ha": float(d.max()), "mean_abs_dE_L_Ha": float(d.mean()),
+                        "max_rel": float(d.max() / Eref)}
+            # (1) energy evaluation: both precisions on the same walkers
+            ev = err(pm["eval"][0], e64)
+            ev["max_abs_dterm_Ha"] = dict(zip(("one_body", "coulomb", "exchange"),
+                                              map(float, np.abs(pm["eval"][1] - p64["eval"][1]).max(axis=0))))
+            ev["E_L_terms_mean_Ha"] = [float(v) for v in p64["eval"][1].real.mean(axis=0)]
+            extra["energy_eval_error"] = ev
+            # (2) k substeps from the same walkers and fields, each in its own precision
+            extra["step_error"] = {f"{k}_substeps": err(pm[k], p64[k]) for k in probe_steps}
+            extra["target_abs_dE_L_Ha"] = 1e-5
+            # (3) the two timed runs' own walkers after the whole block
             dE = np.abs(Wd[:, 0] - Wd64[:, 0])
-            extra["energy_eval_error"] = {
-                "max_abs_dE_L_Ha": float(dEe.max()), "mean_abs_dE_L_Ha": float(dEe.mean()),
-                "max_abs_dterm_Ha": {"one_body": float(dTe[0]), "coulomb": float(dTe[1]), "exchange": float(dTe[2])},
-                "E_L_terms_mean_Ha": [float(v) for v in r64["eval_terms"].real.mean(axis=0)],
-                "target_Ha": 1e-5,
-            }
-            extra["trajectory_error"] = {
-                "max_abs_dE_L_Ha": float(dE.max()), "mean_abs_dE_L_Ha": float(dE.mean()),
-                "dEav_Ha": float(abs(eav[-1] - eav64[-1])), "substeps": args.warmup + args.steps,
+            extra["block_trajectory_error"] = {
+                "max_abs_dE_L_Ha": float(np.nanmax(dE)), "mean_abs_dE_L_Ha": float(np.nanmean(dE)),
+                "Eav_Ha": [float(eav[-1]), float(eav64[-1])], "substeps": args.warmup + args.steps,
             }
             extra["E_L_mean_Ha"] = float(Wd64[:, 0].real.mean())
-        print(f"[bench] fp64_dense: {v64:.1f} walker-steps/s; mixed/fp64 = {value / v64:.2f}x; "
-              f"energy eval {extra.get('energy_eval_error')}; trajectory {extra.get('trajectory_error')}", flush=True)
+        print(f"[bench] fp64_dense: {v64:.1f} walker-steps/s; mixed/fp64 = {value / v64:.2f}x", flush=True)
+        for k_, (n_, ms) in sorted(r64["stats"].items(), key=lambda kv: -kv[1][1])[:12]:
+            print(f"[bench]   fp64 {k_:20s} {n_:6d} launches {ms:10.2f} ms total {ms / max(n_, 1):9.3f} ms avg",
+                  flush=True)
+        for key in ("energy_eval_error", "step_error", "block_trajectory_error"):
+            print(f"[bench] {key}: {extra.get(key)}", flush=True)
 
     cpu = None
     if args.cpu:
@@ -213,7 +241,16 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
-    print(json.dumps(line), flush=True)
+
+    def finite(v):  # strict JSON: non-finite numbers become null
+        if isinstance(v, dict):
+            return {k_: finite(x_) for k_, x_ in v.items()}
+        if isinstance(v, (list, tuple)):
+            return [finite(x_) for x_ in v]
+        if isinstance(v, float) and not math.isfinite(v):
+            return None
+        return v
+    print(json.dumps(finite(line)), flush=True)
 
 
 if __name__ == "__main__":

@@ -87,12 +87,21 @@ typedef enum {
    * memory, [2][NM][NM] in the contraction precision, when it fits, so the
    * two-body energy is one GEMM per spin; 0: always use the Cholesky form. */
   AFQMC_OPT_DENSE_EXCHANGE = 0,
-  /* mixed modes only.  1 (default): the local energy is evaluated from an
-   * FP64 density matrix (rebuilt from the FP64 inverses), with the one-body
-   * and Coulomb terms in FP64 (an FP64 copy of the half-rotated vectors,
-   * 2*K*N*M complex doubles, is kept); the exchange term stays in the
-   * contraction precision.  0: the whole energy in the contraction precision. */
-  AFQMC_OPT_ENERGY_FP64 = 1
+  /* mixed modes only: precision level of the local energy.
+   *   0: the whole energy in the contraction precision;
+   *   1 (default): FP64 density matrix (rebuilt from the FP64 inverses),
+   *      one-body and Coulomb terms in FP64 (an FP64 copy of the half-rotated
+   *      vectors, 2*K*N*M complex doubles, is kept), exchange in the
+   *      contraction precision;
+   *   2: as 1 with the exchange in FP32;
+   *   3: as 1 with the exchange in FP64 (dense exchange block in FP64). */
+  AFQMC_OPT_ENERGY_PRECISION = 1,
+  /* 0 (default): orthogonalise through two Cholesky QR passes (FP64 MFMA
+   * GEMMs) and the sign reconstruction, which reproduce the LAPACK LQ factor;
+   * matrices too ill-conditioned for that route (and N > 128) use the blocked
+   * Householder kernel.  1: always the blocked Householder kernel.  May be
+   * changed at any time. */
+  AFQMC_OPT_ORTHO_HOUSEHOLDER = 2
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

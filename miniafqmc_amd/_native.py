@@ -25,7 +25,7 @@ STATUS = {
 MODES = {"fp64_csr": 0, "fp64_dense": 1, "mixed_fp32": 2, "mixed_bf16": 3}
 
 BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS = range(8)
-OPT_DENSE_EXCHANGE, OPT_ENERGY_FP64 = 0, 1
+OPT_DENSE_EXCHANGE, OPT_ENERGY_PRECISION, OPT_ORTHO_HOUSEHOLDER = 0, 1, 2
 
 # every symbol include/afqmc_hip.h declares
 EXPORTED = [

@@ -23,7 +23,7 @@ def _dp(a):
 
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
-                 use_vakbl: bool = True, dense_exchange: bool = True, energy_fp64: bool = True):
+                 use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
@@ -31,7 +31,7 @@ class AfqmcSys:
         nat.check(nat.lib.afqmc_ctx_create(self.M, self.N, self.K, nwalk, nat.MODES[mode], C.byref(h)))
         self.ctx = h
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DENSE_EXCHANGE, 1.0 if dense_exchange else 0.0))
-        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ENERGY_FP64, 1.0 if energy_fp64 else 0.0))
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ENERGY_PRECISION, float(energy_precision)))
         ta = np.ascontiguousarray(system.trial_alpha)
         tb = np.ascontiguousarray(system.trial_beta)
         sp = system.spvn.view()
@@ -132,6 +132,10 @@ class AfqmcSys:
         self._check(nat.lib.afqmc_weights(self.ctx, dt, C.byref(e)))
         return e.value
 
+    def set_ortho_householder(self, on):
+        """Always use the blocked Householder kernel for the orthogonalisation."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ORTHO_HOUSEHOLDER, 1.0 if on else 0.0))
+
     def orthogonalize(self):
         """afqmc_sys::orthogonalize (afqmc_sys.hpp:175-207)."""
         self._check(nat.lib.afqmc_orthogonalize(self.ctx))
@@ -171,6 +175,9 @@ class AfqmcSys:
         e = C.c_double(0)
         self._check(nat.lib.afqmc_get_eshift(self.ctx, C.byref(e)))
         return e.value
+
+    def set_eshift(self, e):
+        self._check(nat.lib.afqmc_set_eshift(self.ctx, float(e)))
 
     def run(self, nsteps, nsubsteps, northo, dt, first=0, energy=True):
         """The driver's step loop (miniafqmc_base.cpp:248-350); returns Eav per step."""

@@ -68,8 +68,11 @@ struct afqmc_ctx {
   long long v_nnz = 0;
   void* Ld = nullptr;  // dense modes: [K][M*M]  (zd or zf)
   void* Lt = nullptr;  // [2][K][N*M] (zd or zf); scaled by sqrt(dt)
-  zd* Lt64 = nullptr;  // mixed modes with the FP64 energy: Lt in FP64
-  bool energy_fp64 = true;  // mixed modes: density matrix, one-body and Coulomb energy in FP64
+  zd* Lt64 = nullptr;  // mixed modes with energy_level >= 1: Lt in FP64
+  // mixed modes: 0 energy in the contraction precision; 1 FP64 density
+  // matrix, one-body and Coulomb terms; 2 and exchange in FP32; 3 and
+  // exchange in FP64
+  int energy_level = 1;
 
   // walker state and per-substep buffers
   zd* W = nullptr;
@@ -99,12 +102,17 @@ struct afqmc_ctx {
   zd* e_one = nullptr;
   zd* e_v = nullptr;      // [nw][K] direct-term contractions
   void* Ybuf = nullptr;
-  long ybuf_elems = 0;
+  size_t ybuf_bytes = 0;
   // same-spin exchange block of Vakbl, dense [2][NM][NM] (scaled by dt), in
   // the contraction precision; null when it does not fit in device memory
   void* Vx = nullptr;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
+  // orthogonalisation: Cholesky QR scratch (allocated at the first call)
+  bool ortho_householder = false;  // always use the blocked Householder kernel
+  zd* oU = nullptr;   // [2nw][N][N] R^{-1}
+  zd* oX = nullptr;   // [2][2nw][M][N] the two Cholesky QR passes
+  int* ofail = nullptr;  // [2nw] matrices left to the Householder kernel
   zd* Gcloc = nullptr;    // CSR mode Vakbl energy
 
   // profiling
@@ -678,22 +686,113 @@ int do_weights(afqmc_ctx* c, const double* dtp, double dt)
 constexpr int kLqNB = 8;
 size_t lq_lds_bytes(int M) { return sizeof(zd) * ((size_t)kLqNB * M + 128 * kLqNB + kLqNB * kLqNB + 128 + 4 * kLqNB * kLqNB); }
 
+// Cholesky QR declines a matrix whose pivot falls below this fraction of
+// the largest diagonal entry of its Gram matrix (condition number above
+// about 1e4), where its forward error would approach 1e-12.
+constexpr double kCholTol = 1e-8;
+
+// One Cholesky QR pass on the batch Win [2nw][M][N]: S = Win^T conj(Win)
+// (= B^H B for B = conj(Win)), U = R^{-1} from S = R^H R, Wout = Win conj(U)
+// (= conj(B R^{-1})).
+int cholqr_pass(afqmc_ctx* c, const zd* Win, zd* Wout)
+{
+  const int M = c->M, N = c->N, nb = 2 * c->nw;
+  ZGemmArgs a = gargs();
+  a.A = Win;  // A(a, i) = Win[i][a]
+  a.sam = 1;
+  a.sak = N;
+  a.sa1 = (long)M * N;
+  a.B = Win;  // B(b, i) = conj(Win[i][b])
+  a.sbn = 1;
+  a.sbk = N;
+  a.sb1 = (long)M * N;
+  a.conjb = 1;
+  a.C = c->T1;  // S[a][b]
+  a.scm = N;
+  a.scn = 1;
+  a.sc1 = (long)N * N;
+  a.m = N;
+  a.n = N;
+  a.k = M;
+  RET((zgemm<CP_F64, double, double, double, false, false>(c, "ortho_gram", a, nb)));
+  RET(launch(c, "ortho_chol_inv", [&]() {
+    if (N <= 32)
+      hipLaunchKernelGGL((chol_inv_kernel<32, 4>), dim3(nb), dim3(128), 0, c->stream, c->T1, c->oU, N, kCholTol,
+                         c->ofail);
+    else if (N <= 64)
+      hipLaunchKernelGGL((chol_inv_kernel<64, 4>), dim3(nb), dim3(256), 0, c->stream, c->T1, c->oU, N, kCholTol,
+                         c->ofail);
+    else
+      hipLaunchKernelGGL((chol_inv_kernel<128, 8>), dim3(nb), dim3(1024), 0, c->stream, c->T1, c->oU, N, kCholTol,
+                         c->ofail);
+  }));
+  ZGemmArgs b = gargs();
+  b.A = Win;  // A(i, a) = Win[i][a]
+  b.sam = N;
+  b.sak = 1;
+  b.sa1 = (long)M * N;
+  b.B = c->oU;  // B(b, a) = conj(U[a][b]), zero for a > b
+  b.sbn = 1;
+  b.sbk = N;
+  b.sb1 = (long)N * N;
+  b.conjb = 1;
+  b.ktri = 1;
+  b.C = Wout;
+  b.scm = N;
+  b.scn = 1;
+  b.sc1 = (long)M * N;
+  b.m = M;
+  b.n = N;
+  b.k = N;
+  return zgemm<CP_F64, double, double, double, true, false>(c, "ortho_apply", b, nb);
+}
+
+// afqmc_sys::orthogonalize (afqmc_sys.hpp:175-207): the LAPACK LQ factor of
+// every walker-spin matrix, by two Cholesky QR passes and the sign
+// reconstruction; matrices too ill-conditioned for Cholesky QR go through
+// the blocked Householder kernel.
 int do_ortho(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
   c->g_fresh = false;
   const size_t lds = lq_lds_bytes(c->M);
-  return launch(c, "lq_orthogonalize", [&]() {
-    hipLaunchKernelGGL(lq_blocked_kernel<kLqNB>, dim3(2 * c->nw), dim3(256), lds, c->stream, c->W, c->M, c->N);
+  const int M = c->M, N = c->N, nb = 2 * c->nw;
+  if (c->ortho_householder || N > 128) {
+    return launch(c, "lq_orthogonalize", [&]() {
+      hipLaunchKernelGGL(lq_blocked_kernel<kLqNB>, dim3(nb), dim3(256), lds, c->stream, c->W, M, N,
+                         (const int*)nullptr);
+    });
+  }
+  if (!c->oU) {
+    RET(dalloc(c, &c->oU, (long)nb * N * N));
+    RET(dalloc(c, &c->oX, 2L * nb * M * N));
+    RET(dalloc(c, &c->ofail, nb));
+  }
+  zd* X1 = c->oX;
+  zd* X2 = c->oX + (long)nb * M * N;
+  RET(launch(c, "ortho_flags", [&]() { (void)hipMemsetAsync(c->ofail, 0, sizeof(int) * nb, c->stream); }));
+  RET(cholqr_pass(c, c->W, X1));
+  RET(cholqr_pass(c, X1, X2));
+  RET(launch(c, "ortho_sign", [&]() {
+    if (N <= 32)
+      hipLaunchKernelGGL((ortho_sign_kernel<32, 4>), dim3(nb), dim3(128), 0, c->stream, X2, c->W, M, N, c->ofail);
+    else if (N <= 64)
+      hipLaunchKernelGGL((ortho_sign_kernel<64, 4>), dim3(nb), dim3(256), 0, c->stream, X2, c->W, M, N, c->ofail);
+    else
+      hipLaunchKernelGGL((ortho_sign_kernel<128, 8>), dim3(nb), dim3(1024), 0, c->stream, X2, c->W, M, N, c->ofail);
+  }));
+  return launch(c, "lq_fallback", [&]() {
+    hipLaunchKernelGGL(lq_blocked_kernel<kLqNB>, dim3(nb), dim3(256), lds, c->stream, c->W, M, N,
+                       (const int*)c->ofail);
   });
 }
 
 // Local energy from G in [nw][2][N][M] layout (type GT): one-body haj.G,
 // Coulomb term from v = Lv.G (Lv [2][K][NM] of type LVT, scaled by sqrt(dt),
-// contracted in precision CPV) and exchange from the dense block Vx or the
-// Cholesky form with Lt (type LT_), both in precision CP.
+// contracted in precision CPV) and exchange in precision CP, from the dense
+// block Vx (type VXT) or from the Cholesky form with Lx (type LT_).
 template <int CPV, class LVT, int CP, class GT, class LT_, class YT, class VXT>
-int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv)
+int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv, const void* Lx)
 {
   const int M = c->M, N = c->N, K = c->K, nw = c->nw;
   const long NM = (long)N * M;
@@ -709,8 +808,9 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv)
                        inv_dt);
   }));
   if (c->Vx) {
-    // exchange from the dense same-spin block of Vakbl held in HBM:
-    // Z(w, p) = sum_q G_s(w, q) Vx_s(p, q);  e_x(w, s) = sum_p G_s(w, p) Z(w, p)
+    // exchange from the dense same-spin block of Vakbl held in HBM as its
+    // lower triangle with the strictly lower part doubled:
+    // Z(w, p) = sum_{q <= p} G_s(w, q) Vx'_s(p, q);  e_x(w, s) = sum_p G_s(w, p) Z(w, p)
     using ZT = typename ztype<YT>::type;
     for (int sp = 0; sp < 2; sp++) {
       ZGemmArgs a = gargs();
@@ -727,6 +827,7 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv)
       a.n = (int)NM;
       a.k = (int)NM;
       a.ar = inv_dt;
+      a.ktri = 1;
       RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
       RET(launch(c, "energy_exch_dot", [&]() {
         hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream, (int)NM,
@@ -738,11 +839,12 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv)
   // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba, for all
   // walker-spin pairs at once over chunks of nq vectors
   const long per_item = (long)N * N;
-  const int nq = (int)std::max<long>(1, std::min<long>(K, c->ybuf_elems / (2L * nw * per_item)));
+  const long ycap = (long)(c->ybuf_bytes / sizeof(typename ztype<YT>::type));
+  const int nq = (int)std::max<long>(1, std::min<long>(K, ycap / (2L * nw * per_item)));
   for (int n0 = 0; n0 < K; n0 += nq) {
     const int cnt = std::min(nq, K - n0);
     ZGemmArgs a = gargs();
-    a.A = zadv<LT_>(c->Lt, (long)n0 * NM);
+    a.A = zadv<LT_>(Lx, (long)n0 * NM);
     a.sam = M;
     a.sak = 1;
     a.sa1 = 0;
@@ -815,25 +917,32 @@ template <class OT> int build_vx(afqmc_ctx* c, const zd* Lt64)
     a.k = (int)K;
     RET((zgemm<CP_F64, double, double, OT, false, false>(c, "build_vx", a, (int)(N * N))));
   }
-  return AFQMC_OK;
+  // Vx_s is symmetric (swap (a,k) <-> (b,l)): keep the doubled lower triangle
+  using ZT = typename ztype<OT>::type;
+  return launch(c, "vx_lower", [&]() {
+    hipLaunchKernelGGL((sym_lower_kernel<ZT>), dim3(nblk(2 * NM * NM, 256)), dim3(256), 0, c->stream, (ZT*)c->Vx, NM,
+                       2L);
+  });
 }
 
 // Build Vx when it fits in device memory with room to spare; otherwise the
 // energy uses the Cholesky form.
-template <class ZT> int maybe_vx(afqmc_ctx* c, const zd* Lt64)
+// ZT: storage of Vx; the exchange GEMM writes Z [nw][NM] of zsize bytes per element.
+template <class ZT> int maybe_vx(afqmc_ctx* c, const zd* Lt64, size_t zsize)
 {
   if (!c->allow_vx) return AFQMC_OK;
   const long NM = (long)c->N * c->M, nw = c->nw;
   const size_t bytes = sizeof(ZT) * 2 * (size_t)NM * NM;
+  const size_t ybytes = zsize * (size_t)nw * NM;
   size_t fr = 0, tot = 0;
   HIPCHK(c, hipMemGetInfo(&fr, &tot));
-  const size_t need = bytes + sizeof(ZT) * (size_t)nw * NM + ((size_t)8 << 30);
+  const size_t need = bytes + ybytes + ((size_t)8 << 30);
   if (need > fr) return AFQMC_OK;
   RET(dalloc_bytes(c, &c->Vx, bytes));
-  if ((long)c->ybuf_elems < nw * NM) {
+  if (c->ybuf_bytes < ybytes) {
     dfree(c, c->Ybuf);
-    c->ybuf_elems = nw * NM;
-    RET(dalloc_bytes(c, &c->Ybuf, sizeof(ZT) * c->ybuf_elems));
+    c->ybuf_bytes = ybytes;
+    RET(dalloc_bytes(c, &c->Ybuf, ybytes));
   }
   if (std::is_same<ZT, zd>::value) return build_vx<double>(c, Lt64);
   if (std::is_same<ZT, zb>::value) return build_vx<bfs>(c, Lt64);
@@ -873,20 +982,27 @@ int do_energy(afqmc_ctx* c, double* eav_out)
         hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->G, c->Gtmp,
                            (int)NAK, nw, (long)nw, NAK);
       }));
-      RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->Gtmp, c->Lt)));
+      RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->Gtmp, c->Lt, c->Lt)));
     }
   } else if (c->mode == AFQMC_FP64_DENSE) {
-    RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->G, c->Lt)));
+    RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, c->G, c->Lt, c->Lt)));
   } else if (c->Lt64) {
-    // FP64 density matrix, one-body and Coulomb terms; exchange in the
-    // contraction precision
+    // FP64 density matrix, one-body and Coulomb terms; the exchange in FP64
+    // (level 3), FP32 (level 2) or the contraction precision (level 1)
     RET(g_fp64(c));
-    if (c->bsplit) RET((energy_cholesky<CP_F64, double, CP_BF16X3, double, bfs, float, bfs>(c, c->Gtmp, c->Lt64)));
-    else RET((energy_cholesky<CP_F64, double, CP_F32, double, float, float, float>(c, c->Gtmp, c->Lt64)));
+    const void *G = c->Gtmp, *L64 = c->Lt64;
+    if (c->energy_level >= 3)
+      RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, G, L64, L64)));
+    else if (c->energy_level == 2)
+      RET((energy_cholesky<CP_F64, double, CP_F32, double, double, float, float>(c, G, L64, L64)));
+    else if (c->bsplit)
+      RET((energy_cholesky<CP_F64, double, CP_BF16X3, double, bfs, float, bfs>(c, G, L64, c->Lt)));
+    else
+      RET((energy_cholesky<CP_F64, double, CP_F32, double, float, float, float>(c, G, L64, c->Lt)));
   } else if (c->mode == AFQMC_MIXED_FP32) {
-    RET((energy_cholesky<CP_F32, float, CP_F32, float, float, float, float>(c, c->G, c->Lt)));
+    RET((energy_cholesky<CP_F32, float, CP_F32, float, float, float, float>(c, c->G, c->Lt, c->Lt)));
   } else {
-    RET((energy_cholesky<CP_BF16X3, bfs, CP_BF16X3, bfs, bfs, float, bfs>(c, c->G, c->Lt)));
+    RET((energy_cholesky<CP_BF16X3, bfs, CP_BF16X3, bfs, bfs, float, bfs>(c, c->G, c->Lt, c->Lt)));
   }
   if (eav_out) {
     std::vector<zd> wd((size_t)nw * 8);
@@ -1033,8 +1149,8 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   chk(dalloc(c, &c->e_exch_part, 2 * nw));
   chk(dalloc(c, &c->e_one, nw));
   chk(dalloc(c, &c->e_v, nw * K));
-  c->ybuf_elems = std::max<long>(2 * nw * N * N, std::min<long>(2 * nw * K * N * N, 1L << 25));
-  chk(dalloc_bytes(c, &c->Ybuf, c->ybuf_elems * zs));
+  c->ybuf_bytes = std::max<long>(2 * nw * N * N, std::min<long>(2 * nw * K * N * N, 1L << 25)) * sizeof(zd);
+  chk(dalloc_bytes(c, &c->Ybuf, c->ybuf_bytes));
   if (r != AFQMC_OK) {
     afqmc_ctx_destroy(c);
     return r;
@@ -1075,10 +1191,18 @@ int afqmc_sync(afqmc_ctx* c)
 int afqmc_set_option(afqmc_ctx* c, int option, double value)
 {
   if (!c) return AFQMC_ERR_ARG;
-  if (option == AFQMC_OPT_DENSE_EXCHANGE || option == AFQMC_OPT_ENERGY_FP64) {
+  if (option == AFQMC_OPT_DENSE_EXCHANGE || option == AFQMC_OPT_ENERGY_PRECISION) {
     if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
-    if (option == AFQMC_OPT_DENSE_EXCHANGE) c->allow_vx = value != 0.0;
-    else c->energy_fp64 = value != 0.0;
+    if (option == AFQMC_OPT_DENSE_EXCHANGE) {
+      c->allow_vx = value != 0.0;
+    } else {
+      if (value < 0 || value > 3 || value != (int)value) return fail(c, AFQMC_ERR_ARG, "energy precision level is 0..3");
+      c->energy_level = (int)value;
+    }
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {
+    c->ortho_householder = value != 0.0;
     return AFQMC_OK;
   }
   return fail(c, AFQMC_ERR_ARG, "unknown option");
@@ -1213,15 +1337,15 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     } else {
       c->Lt = Lt64;  // two-body energy from the vectors
       RET(dalloc(c, &c->Gtmp, 2 * NM * c->nw));
-      RET(maybe_vx<zd>(c, Lt64));
+      RET(maybe_vx<zd>(c, Lt64, sizeof(zd)));
     }
   } else {
     dfree(c, rp);
     dfree(c, col);
     dfree(c, val);
-    if (c->bsplit) RET(maybe_vx<zb>(c, Lt64));
-    else if (c->mixed) RET(maybe_vx<zf>(c, Lt64));
-    else RET(maybe_vx<zd>(c, Lt64));
+    if (!c->mixed || c->energy_level >= 3) RET(maybe_vx<zd>(c, Lt64, sizeof(zd)));
+    else if (c->bsplit && c->energy_level <= 1) RET(maybe_vx<zb>(c, Lt64, sizeof(zf)));
+    else RET(maybe_vx<zf>(c, Lt64, sizeof(zf)));
     if (c->mixed) {
       zf* Ld32 = nullptr;
       zf* Lt32 = nullptr;
@@ -1241,7 +1365,7 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       }));
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Ld64);
-      if (c->energy_fp64) {
+      if (c->energy_level >= 1) {
         c->Lt64 = Lt64;
         RET(dalloc(c, &c->Gtmp, 2 * NM * c->nw));
       } else {

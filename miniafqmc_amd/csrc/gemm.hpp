@@ -46,6 +46,11 @@ struct ZGemmArgs {
   const void* F;
   // tile grid: blockIdx.x runs over MT*NT*batch tiles (filled by the launcher)
   int MT, NT;
+  // lower-triangular B: B(n, k) = 0 for k > n, so the tile of columns
+  // [n0, n0 + BNC) stops its k loop at n0 + BNC; heavy tiles run first
+  int ktri;
+  // use conj(B) in place of B
+  int conjb;
 };
 
 template <int CP> struct LdsT;
@@ -109,11 +114,22 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   long lin = blockIdx.x;
   {
     const long T = gridDim.x, per = T / 8;
-    if (per > 0 && lin < per * 8) lin = (lin % 8) * per + lin / 8;
+    if (p.ktri) {
+      // triangular work falls with the N tile: deal N tiles round-robin to
+      // the XCDs (heaviest first) and keep the M tiles of one N tile on one XCD
+      if ((p.NT & 7) == 0 && T == (long)p.MT * p.NT) {
+        const long x = lin & 7, q = lin >> 3;
+        lin = (x + 8 * (q / p.MT)) * p.MT + q % p.MT;
+      }
+    } else if (per > 0 && lin < per * 8) {
+      lin = (lin % 8) * per + lin / 8;
+    }
   }
   const int mt = (int)(lin % p.MT);
   const long rest = lin / p.MT;
-  const int m0 = mt * BM, n0 = (int)(rest % p.NT) * BNC;
+  const int ntl = (int)(rest % p.NT);
+  const int m0 = mt * BM, n0 = (p.ktri ? p.NT - 1 - ntl : ntl) * BNC;
+  const int kend = p.ktri ? min(p.k, n0 + BNC) : p.k;
   const int b = (int)(rest / p.NT);
   const long bq = b / p.bdiv, br_ = b % p.bdiv;
   const AZ* __restrict__ A = (const AZ*)p.A + bq * p.sa1 + br_ * p.sa2;
@@ -144,7 +160,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int row, kc;
       if (AK) { row = e / BKC; kc = e % BKC; } else { kc = e / BM; row = e % BM; }
       int gm = m0 + row, gk = k0 + kc;
-      if (e < BM * BKC && gm < p.m && gk < p.k) ra[i] = A[(long)gm * p.sam + (long)gk * p.sak];
+      if (e < BM * BKC && gm < p.m && gk < kend) ra[i] = A[(long)gm * p.sam + (long)gk * p.sak];
       else zero_z(ra[i]);
     }
 #pragma unroll
@@ -153,7 +169,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int col, kc;
       if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
       int gn = n0 + col, gk = k0 + kc;
-      if (e < BNC * BKC && gn < p.n && gk < p.k) rb[i] = B[(long)gn * p.sbn + (long)gk * p.sbk];
+      if (e < BNC * BKC && gn < p.n && gk < kend) rb[i] = B[(long)gn * p.sbn + (long)gk * p.sbk];
       else zero_z(rb[i]);
     }
   };
@@ -184,21 +200,25 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int col, kc;
       if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
       if constexpr (CP == CP_F64) {
-        sB[st][0][col * LDK + kc] = zd{(double)rb[i].re, (double)rb[i].im};
+        const double bi = (double)rb[i].im;
+        sB[st][0][col * LDK + kc] = zd{(double)rb[i].re, p.conjb ? -bi : bi};
       } else if constexpr (CP == CP_F32) {
-        sB[st][0][col * LDK + kc] = zf{(float)rb[i].re, (float)rb[i].im};
+        const float bi = (float)rb[i].im;
+        sB[st][0][col * LDK + kc] = zf{(float)rb[i].re, p.conjb ? -bi : bi};
       } else if constexpr (std::is_same<BZ, zb>::value) {
-        sB[st][0][col * LDK + kc] = rb[i].hi;
-        sB[st][1][col * LDK + kc] = rb[i].lo;
+        const uint32_t flip = p.conjb ? 0x80000000u : 0u;  // sign of the imaginary halves
+        sB[st][0][col * LDK + kc] = rb[i].hi ^ flip;
+        sB[st][1][col * LDK + kc] = rb[i].lo ^ flip;
       } else {
-        const zb v = split_z((float)rb[i].re, (float)rb[i].im);
+        const float bi = (float)rb[i].im;
+        const zb v = split_z((float)rb[i].re, p.conjb ? -bi : bi);
         sB[st][0][col * LDK + kc] = v.hi;
         sB[st][1][col * LDK + kc] = v.lo;
       }
     }
   };
 
-  const int nk = (p.k + BKC - 1) / BKC;
+  const int nk = (kend + BKC - 1) / BKC;
   gload(0);
   sstore(0);
   __syncthreads();

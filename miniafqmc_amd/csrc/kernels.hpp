@@ -232,6 +232,39 @@ __global__ void zconvert_kernel(const IT* __restrict__ in, OT* __restrict__ out,
   }
 }
 
+__device__ inline zd ztwice(zd v) { return zd{2.0 * v.re, 2.0 * v.im}; }
+__device__ inline zf ztwice(zf v) { return zf{2.0f * v.re, 2.0f * v.im}; }
+__device__ inline uint32_t bf2_twice(uint32_t w)
+{
+  const unsigned short a = f2bf(2.0f * bf2f((unsigned short)(w & 0xffff)));
+  const unsigned short b = f2bf(2.0f * bf2f((unsigned short)(w >> 16)));
+  return (uint32_t)a | ((uint32_t)b << 16);
+}
+__device__ inline zb ztwice(zb v) { return zb{bf2_twice(v.hi), bf2_twice(v.lo)}; }
+
+// nmat symmetric n x n matrices -> their lower triangles with the strictly
+// lower part doubled, upper part zero, so that
+//   sum_pq g_p V_pq g_q = sum_p g_p sum_{q <= p} V'_pq g_q
+// (half the work of the full quadratic form).  Scaling by 2 is exact in
+// every storage format.
+template <class ZT>
+__global__ void sym_lower_kernel(ZT* __restrict__ V, long n, long nmat)
+{
+  const long tot = nmat * n * n;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long st = (long)gridDim.x * blockDim.x;
+  for (; i < tot; i += st) {
+    const long col = i % n, row = (i / n) % n;
+    if (col > row) {
+      ZT z;
+      zero_z(z);
+      V[i] = z;
+    } else if (col < row) {
+      V[i] = ztwice(V[i]);
+    }
+  }
+}
+
 // y += x (complex, FP64)
 __global__ void zaxpy1_kernel(const zd* __restrict__ x, zd* __restrict__ y, long n)
 {
@@ -562,9 +595,13 @@ __device__ inline void trailing_update(zd* W, int N, int rb, int c0, int LP, con
   }
 }
 
+// only != null: matrices b with only[b] == 0 are skipped (the fallback pass
+// after the Cholesky route).
 template <int NB>
-__global__ __launch_bounds__(256) void lq_blocked_kernel(zd* __restrict__ Wall, int M, int N)
+__global__ __launch_bounds__(256) void lq_blocked_kernel(zd* __restrict__ Wall, int M, int N,
+                                                         const int* __restrict__ only)
 {
+  if (only && only[blockIdx.x] == 0) return;
   extern __shared__ zd sm[];
   zd* P = sm;                  // [NB][M]
   zd* W1 = P + NB * M;         // [128][NB]
@@ -641,6 +678,192 @@ __global__ __launch_bounds__(256) void lq_blocked_kernel(zd* __restrict__ Wall, 
       W[(long)c * N + i0 + r] = zd{0, 0};
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Orthogonalisation through Cholesky QR.  For B = conj(W_s) (M x N), the
+// reference's zgelqf + zunglq on the column-major view W_s^T equals
+// zgeqrf + zungqr on B, so the new walker is conj(Q_h) with B = Q_h R_h and
+// R_h's diagonal real with LAPACK's sign rule (beta = -sign(Re alpha) |x|).
+// With the positive factorisation B = Q+ R+ (Cholesky of B^H B), Q_h = Q+ D
+// for a diagonal D of +-1.  D is recovered without the reflectors: the
+// no-pivot LU of the top N x N block X of Q+ in which each pivot is first
+// shifted by s_k = -sign(Re X_kk) (the Householder reconstruction of Ballard
+// et al., "Reconstructing Householder vectors from tall-skinny QR", 2015)
+// yields exactly D_k = s_k, because that shift is the reconstructed
+// reflector's choice of beta and the Householder factorisation of B is
+// unique once the signs are fixed.
+
+// Cholesky S = R^H R (R upper, positive diagonal) of a batch of Hermitian
+// N x N matrices (row-major, upper triangle read) followed by the in-place
+// Gauss-Jordan inverse of R: writes U = R^{-1} (upper, row-major).  Thread
+// (i, cg) holds row i, columns cg + CG*t in registers.  A pivot below
+// tol * max_k S_kk marks the matrix in fail[b] (its U is then undefined).
+template <int NMAX, int CG>
+__global__ __launch_bounds__(NMAX* CG) void chol_inv_kernel(const zd* __restrict__ S, zd* __restrict__ U, int N,
+                                                            double tol, int* __restrict__ fail)
+{
+  constexpr int NC = NMAX / CG;
+  __shared__ zd prow[2][NMAX];
+  __shared__ zd colk[2][NMAX];
+  __shared__ double dmax;
+  const int tid = threadIdx.x;
+  const int i = tid / CG, cg = tid % CG;
+  const long b = blockIdx.x;
+  const zd* Sb = S + b * (long)N * N;
+  zd x[NC];
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    const int j = cg + CG * t;
+    x[t] = (i < N && j < N && j >= i) ? Sb[(long)i * N + j] : zd{(i == j) ? 1.0 : 0.0, 0.0};
+  }
+  if (tid == 0) {
+    double m = 0.0;
+    for (int k = 0; k < N; k++) m = fmax(m, Sb[(long)k * N + k].re);
+    dmax = m;
+  }
+  __syncthreads();
+  const double thr = tol * dmax;
+  bool bad = false;
+  // Cholesky, right-looking on the upper triangle
+  for (int k = 0; k < N; k++) {
+    const int tk = k / CG, ck = k % CG, st = k & 1;
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) prow[st][cg + CG * t] = x[t];
+    }
+    __syncthreads();
+    const double d = prow[st][k].re;
+    if (!(d > thr)) {
+      bad = true;
+      break;
+    }
+    const double r = sqrt(d), rinv = 1.0 / r;
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        x[t] = (j < k) ? zd{0, 0} : ((j == k) ? zd{r, 0.0} : zd{x[t].re * rinv, x[t].im * rinv});
+      }
+    } else if (i > k && i < N) {
+      // conj(R_ki) R_kj = conj(S_ki) S_kj / d
+      const zd ski = prow[st][i];
+      const zd f{ski.re / d, -ski.im / d};
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        if (j >= i && j < N) x[t] = zsub(x[t], zmul(f, prow[st][j]));
+        if ((t & 7) == 7) asm volatile("" ::: "memory");  // bound the LDS loads in flight
+      }
+    }
+    (void)tk;
+    (void)ck;
+  }
+  if (bad) {
+    if (tid == 0) fail[b] = 1;
+    return;
+  }
+  // rows i < N now hold R (upper); rows >= N the identity.  In-place
+  // Gauss-Jordan inverse, pivots on the diagonal.
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    const int j = cg + CG * t;
+    if (j < i) x[t] = zd{0, 0};
+  }
+  for (int k = 0; k < N; k++) {
+    const int tk = k / CG, ck = k % CG, st = k & 1;
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) prow[st][cg + CG * t] = x[t];
+    }
+    if (cg == ck) {
+#pragma unroll
+      for (int t = 0; t < NC; t++)
+        if (t == tk) colk[st][i] = x[t];
+    }
+    __syncthreads();
+    const double ipv = 1.0 / prow[st][k].re;  // R_kk is real
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        x[t] = (j == k) ? zd{ipv, 0.0} : zd{x[t].re * ipv, x[t].im * ipv};
+      }
+    } else if (i < k) {
+      const zd g = colk[st][i];
+      const zd gq{-g.re * ipv, -g.im * ipv};
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        if (j == k) x[t] = gq;
+        else x[t] = zadd(x[t], zmul(gq, prow[st][j]));
+        if ((t & 7) == 7) asm volatile("" ::: "memory");
+      }
+    }
+  }
+  zd* Ub = U + b * (long)N * N;
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    const int j = cg + CG * t;
+    if (i < N && j < N) Ub[(long)i * N + j] = x[t];
+  }
+}
+
+// Signs D of the LAPACK factor from the shifted no-pivot LU of the top
+// N x N block of Q+ (conjugation leaves the real parts of the pivots
+// unchanged, so the walker-layout block conj(Q+) is used as is), then
+// Wout = Win D for every row.  Matrices with fail[b] set are skipped.
+template <int NMAX, int CG>
+__global__ __launch_bounds__(NMAX* CG) void ortho_sign_kernel(const zd* __restrict__ Win, zd* __restrict__ Wout,
+                                                              int M, int N, const int* __restrict__ fail)
+{
+  constexpr int NC = NMAX / CG;
+  __shared__ zd prow[2][NMAX];
+  __shared__ zd colk[2][NMAX];
+  __shared__ double sgn[NMAX];
+  const long b = blockIdx.x;
+  if (fail[b]) return;
+  const int tid = threadIdx.x;
+  const int i = tid / CG, cg = tid % CG;
+  const zd* Wb = Win + b * (long)M * N;
+  zd x[NC];
+#pragma unroll
+  for (int t = 0; t < NC; t++) {
+    const int j = cg + CG * t;
+    x[t] = (i < N && j < N) ? Wb[(long)i * N + j] : zd{0, 0};
+  }
+  for (int k = 0; k < N; k++) {
+    const int tk = k / CG, ck = k % CG, st = k & 1;
+    if (i == k) {
+#pragma unroll
+      for (int t = 0; t < NC; t++) prow[st][cg + CG * t] = x[t];
+    }
+    if (cg == ck) {
+#pragma unroll
+      for (int t = 0; t < NC; t++)
+        if (t == tk) colk[st][i] = x[t];
+    }
+    __syncthreads();
+    const zd pk = prow[st][k];
+    const double sk = (pk.re >= 0.0) ? -1.0 : 1.0;
+    if (tid == 0) sgn[k] = sk;
+    if (i > k && i < N) {
+      const zd l = zdiv(colk[st][i], zd{pk.re - sk, pk.im});
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        const int j = cg + CG * t;
+        if (j > k) x[t] = zsub(x[t], zmul(l, prow[st][j]));
+      }
+    }
+  }
+  __syncthreads();
+  zd* Wo = Wout + b * (long)M * N;
+  const long tot = (long)M * N;
+  for (long e = tid; e < tot; e += NMAX * CG) {
+    const double s = sgn[e % N];
+    const zd v = Wb[e];
+    Wo[e] = zd{s * v.re, s * v.im};
   }
 }
 

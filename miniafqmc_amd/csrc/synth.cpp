@@ -66,55 +66,7 @@ template <class F> void parallel_for(long n, F&& f)
   for (auto& x : th) x.join();
 }
 
-enum Stream : uint64_t { S_H = 1, S_PSI_A = 2, S_PSI_B = 3, S_L = 4, S_X = 5 };
-
-// C = A*B, all square n x n row-major
-void matmul(int n, const cd* A, const cd* B, cd* C)
-{
-  parallel_for(n, [&](long i) {
-    cd* Ci = C + (size_t)i * n;
-    for (int j = 0; j < n; j++) Ci[j] = 0.0;
-    for (int l = 0; l < n; l++) {
-      cd a = A[(size_t)i * n + l];
-      const cd* Bl = B + (size_t)l * n;
-      for (int j = 0; j < n; j++) Ci[j] += a * Bl[j];
-    }
-  });
-}
-
-// expm by scaling and squaring of a Taylor series, FP64.
-void expm(int n, const cd* A, cd* E)
-{
-  double nrm = 0.0;  // max column 1-norm
-  for (int j = 0; j < n; j++) {
-    double s = 0.0;
-    for (int i = 0; i < n; i++) s += std::abs(A[(size_t)i * n + j]);
-    nrm = std::max(nrm, s);
-  }
-  int sq = 0;
-  while (nrm / std::ldexp(1.0, sq) > 0.25) sq++;
-  double f = std::ldexp(1.0, -sq);
-  std::vector<cd> As((size_t)n * n), T((size_t)n * n), T2((size_t)n * n);
-  for (size_t i = 0; i < As.size(); i++) As[i] = A[i] * f;
-  for (size_t i = 0; i < As.size(); i++) E[i] = 0.0;
-  for (int i = 0; i < n; i++) E[(size_t)i * n + i] = 1.0;
-  T.assign(E, E + (size_t)n * n);
-  for (int k = 1; k <= 30; k++) {
-    matmul(n, As.data(), T.data(), T2.data());
-    double tn = 0.0;
-    for (size_t i = 0; i < T2.size(); i++) {
-      T2[i] /= double(k);
-      tn = std::max(tn, std::abs(T2[i]));
-      E[i] += T2[i];
-    }
-    std::swap(T, T2);
-    if (tn < 1e-20) break;
-  }
-  for (int s = 0; s < sq; s++) {
-    matmul(n, E, E, T.data());
-    std::copy(T.begin(), T.end(), E);
-  }
-}
+enum Stream : uint64_t { S_H = 1, S_PSI_B = 3, S_L = 4, S_X = 5 };
 
 // N orthonormal columns (modified Gram-Schmidt, two passes) of a Gaussian M x N
 void random_orbitals(int M, int N, uint64_t seed, uint64_t stream, cd* Psi)
@@ -164,25 +116,55 @@ int afqmc_synth_create(const afqmc_synth_params* prm, afqmc_synth** out)
   const uint64_t seed = prm->seed;
   const size_t MN = (size_t)M * N, MM = (size_t)M * M;
 
-  // one-body h = (A + A^H)/2, A complex Gaussian with entries of std sigma_h
-  std::vector<cd> h(MM);
+  // one-body h = U diag(lambda) U^H with U a random unitary (Gram-Schmidt of
+  // a complex Gaussian) and lambda ascending at the quantiles of the
+  // semicircle law of radius sigma_h*sqrt(2M) (the spectrum of (A + A^H)/2
+  // for A complex Gaussian with entries of std sigma_h).  The trial orbitals
+  // are the first N columns of U, i.e. the N lowest eigenvectors of h, so the
+  // walk projects from a determinant close to the ground state; with a
+  // random determinant the phaseless walk loses every walker's weight
+  // within ~100 substeps.
+  std::vector<cd> U(MM);
+  random_orbitals(M, M, seed, S_H, U.data());
+  std::vector<double> lam(M);
   {
-    std::vector<cd> A(MM);
-    for (size_t i = 0; i < MM; i++) {
-      double a, b;
-      normal2(seed, S_H, i, a, b);
-      A[i] = cd(a, b) * (prm->sigma_h / std::sqrt(2.0));
+    const double R = prm->sigma_h * std::sqrt(2.0 * M);
+    for (int i = 0; i < M; i++) {
+      // F(x) = 1/2 + (x sqrt(1-x^2) + asin x)/pi = (i + 1/2)/M by bisection
+      const double target = (i + 0.5) / M;
+      double lo = -1.0, hi = 1.0;
+      for (int it = 0; it < 100; it++) {
+        const double x = 0.5 * (lo + hi);
+        const double F = 0.5 + (x * std::sqrt(1.0 - x * x) + std::asin(x)) / M_PI;
+        if (F < target) lo = x;
+        else hi = x;
+      }
+      lam[i] = R * 0.5 * (lo + hi);
     }
-    for (int i = 0; i < M; i++)
-      for (int j = 0; j < M; j++) h[(size_t)i * M + j] = 0.5 * (A[(size_t)i * M + j] + std::conj(A[(size_t)j * M + i]));
   }
+  std::vector<cd> h(MM);
+  parallel_for(M, [&](long i) {
+    for (int j = 0; j < M; j++) {
+      cd acc = 0.0;
+      for (int k = 0; k < M; k++) acc += U[(size_t)i * M + k] * lam[k] * std::conj(U[(size_t)j * M + k]);
+      h[(size_t)i * M + j] = acc;
+    }
+  });
+  for (int i = 0; i < M; i++) h[(size_t)i * M + i] = cd(h[(size_t)i * M + i].real(), 0.0);
   s->h1.resize(2 * MM);
   std::memcpy(s->h1.data(), h.data(), MM * sizeof(cd));
 
-  // trial orbitals Psi_a, Psi_b; trialwfn = conj(Psi)
-  std::vector<cd> pa(MN), pb(MN);
-  random_orbitals(M, N, seed, S_PSI_A, pa.data());
-  random_orbitals(M, N, seed, S_PSI_B, pb.data());
+  // trial orbitals: Psi_a = U[:, :N]; Psi_b = Psi_a V with V a random N x N
+  // unitary (the same determinant, different orbitals); trialwfn = conj(Psi)
+  std::vector<cd> pa(MN), pb(MN), V((size_t)N * N);
+  random_orbitals(N, N, seed, S_PSI_B, V.data());
+  for (int i = 0; i < M; i++)
+    for (int a = 0; a < N; a++) {
+      pa[(size_t)i * N + a] = U[(size_t)i * M + a];
+      cd acc = 0.0;
+      for (int b = 0; b < N; b++) acc += U[(size_t)i * M + b] * V[(size_t)b * N + a];
+      pb[(size_t)i * N + a] = acc;
+    }
   s->trial_a.resize(2 * MN);
   s->trial_b.resize(2 * MN);
   cd* ta = reinterpret_cast<cd*>(s->trial_a.data());
@@ -204,12 +186,19 @@ int afqmc_synth_create(const afqmc_synth_params* prm, afqmc_synth** out)
       }
   }
 
-  // Propg1 = expm(-dt/2 h)
+  // Propg1 = expm(-dt/2 h) = U diag(exp(-dt/2 lambda)) U^H
   {
-    std::vector<cd> A(MM);
-    for (size_t i = 0; i < MM; i++) A[i] = -0.5 * prm->dt * h[i];
     s->propg1.resize(2 * MM);
-    expm(M, A.data(), reinterpret_cast<cd*>(s->propg1.data()));
+    cd* P = reinterpret_cast<cd*>(s->propg1.data());
+    std::vector<double> e(M);
+    for (int k = 0; k < M; k++) e[k] = std::exp(-0.5 * prm->dt * lam[k]);
+    parallel_for(M, [&](long i) {
+      for (int j = 0; j < M; j++) {
+        cd acc = 0.0;
+        for (int k = 0; k < M; k++) acc += U[(size_t)i * M + k] * e[k] * std::conj(U[(size_t)j * M + k]);
+        P[(size_t)i * M + j] = acc;
+      }
+    });
   }
 
   // Cholesky vectors L(ik, n): re, im ~ N(0, sigma^2), kept if |L| > tau with

@@ -144,15 +144,23 @@ def test_phases_fp64_csr(c1):
     g.close()
 
 
+@pytest.mark.parametrize("householder", [False, True])
 @pytest.mark.parametrize("M,N", [(100, 24), (300, 100), (512, 128)])
-def test_orthogonalize_sizes(require_gpu, M, N):
-    """Blocked Householder LQ (ragged panels, up to 512 x 128) vs the oracle's zgelq2/zungl2."""
-    say("orthogonalize", M, N)
+def test_orthogonalize_sizes(require_gpu, M, N, householder):
+    """LAPACK LQ factor (Cholesky QR + sign reconstruction, or the blocked
+    Householder kernel with ragged panels) up to 512 x 128 vs the oracle's
+    zgelq2/zungl2, on Gaussian matrices and on an ill-conditioned one that
+    the Cholesky route hands to the Householder kernel."""
+    say("orthogonalize", M, N, "householder" if householder else "cholesky qr")
     s = m.make_system(M, N, 2, density=1.0, sigma_L=0.01, seed=M)
     nw = 3
     g = m.AfqmcSys(s, nw, "fp64_csr", dense_exchange=False)
+    g.set_ortho_householder(householder)
     r = np.random.default_rng(M)
     W = r.standard_normal((nw, 2, M, N)) + 1j * r.standard_normal((nw, 2, M, N))
+    # nearly parallel columns (condition ~1e5): the Cholesky route declines
+    # this matrix and the Householder kernel takes it
+    W[1, 0, :, 1] = W[1, 0, :, 0] + 5e-5 * (r.standard_normal(M) + 1j * r.standard_normal(M))
     g.set(nat.BUF_W, W)
     g.orthogonalize()
     close(g.W, orc.orthogonalize(W), 1e-10, "orthogonalized walkers")
@@ -253,11 +261,9 @@ def test_energy_terms_sum_and_one_body(c1):
 
 @pytest.mark.parametrize("mode", ["mixed_fp32", "mixed_bf16"])
 def test_mixed_energy_evaluation_error(c1, mode):
-    """Per-walker E_L of the mixed modes on identical walkers vs the FP64 oracle.
-
-    Default (FP64 density matrix, one-body and Coulomb; exchange in the
-    contraction precision): |dE_L| < 1e-5 Ha.  Whole energy in the contraction
-    precision: reported, bounded at 1e-3 Ha.
+    """Per-walker E_L of the mixed modes on identical walkers vs the FP64 oracle,
+    for each energy precision level (0: all in the contraction precision; 1:
+    FP64 G, one-body and Coulomb; 2: and FP32 exchange; 3: and FP64 exchange).
     """
     s, p = c1
     nw = p["nwalk"]
@@ -268,13 +274,13 @@ def test_mixed_energy_evaluation_error(c1, mode):
     Go, _ = orc.calculate_mixed_density_matrix(s, W, Wdo)
     orc.calculate_energy(s, Wdo, Go)
     eo = Wdo[:, 0]
-    for fp64, tol in ((True, 1e-5), (False, 1e-3)):
-        g = m.AfqmcSys(s, nw, mode, energy_fp64=fp64)
+    for level, tol in ((0, 1e-3), (1, 1e-5), (2, 1e-5), (3, 1e-10)):
+        g = m.AfqmcSys(s, nw, mode, energy_precision=level)
         g.set(nat.BUF_W, W)
         g.set(nat.BUF_WDATA, wd)
         g.calculate_mixed_density_matrix()
         g.calculate_energy()
         dE = np.abs(g.W_data[:, 0] - eo)
-        say(f"{mode} energy_fp64={fp64}: max |dE_L| = {dE.max():.3e} Ha (E_L ~ {eo.real.mean():.3f} Ha)")
+        say(f"{mode} energy level {level}: max |dE_L| = {dE.max():.3e} Ha (E_L ~ {eo.real.mean():.3f} Ha)")
         assert dE.max() < tol
         g.close()
