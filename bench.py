@@ -35,6 +35,14 @@ MODE_PEAK = {"fp64_csr": "fp64", "fp64_dense": "fp64", "mixed_fp32": "fp32", "mi
 MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128", "mixed_bf16": "bf16x3+c128"}
 
 
+# HBM bytes per launch measured with rocprofv3 (raw TCC read/write request
+# counters, scaled by a 1 GiB copy), per (call site, workload, mode);
+# profiles/walker_step_N128_M512_hbm_traffic.md has the commands and figures.
+MEASURED_TRAFFIC = {
+    ("prop_taylor", "N128_M512", "mixed_bf16"): 6.51e9,
+}
+
+
 def kernel_flops(name, M, N, K, nw):
     """Algorithmic real flops per launch (8 per complex multiply-add)."""
     c = {
@@ -65,6 +73,9 @@ def main():
     ap.add_argument("--cpu-walkers", type=int, default=6)
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB and exit (reference for "
+                         "hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -72,8 +83,22 @@ def main():
               "path is a design note, not built (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy_mib > 0:
+        # reference for hardware byte counters: one device-to-device copy of
+        # known size, in a process of its own (nothing else is run)
+        import torch
+        n = args.calib_copy_mib << 18  # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written",
+              flush=True)
+        return
+
     import miniafqmc_amd as m
     from miniafqmc_amd import _native as nat
+
 
     t0 = time.time()
     s, p = m.preset_system(args.workload)
@@ -160,8 +185,8 @@ def main():
         ach = flops / avg_s / 1e12
         pk = PEAKS[MODE_PEAK[args.mode] if name not in ("ovlp_gemm", "gc_gemm", "gj_inverse") else "fp64"]
         roof = {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": pk, "unit": "TFLOP/s",
-                "frac": round(ach / pk, 4), "traffic": None, "avg_ms": round(ms / nl, 4),
-                "flops_per_launch": flops}
+                "frac": round(ach / pk, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
+                "avg_ms": round(ms / nl, 4), "flops_per_launch": flops}
 
     extra = {}
     if args.fp64 and args.mode != "fp64_dense":

@@ -15,6 +15,8 @@
 #include "gemm.hpp"
 #include "kernels.hpp"
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -195,8 +197,40 @@ hipEvent_t get_event(afqmc_ctx* c)
 }
 
 // Launch wrapper: brackets the launch with events when profiling.
+// Optional ROCTx ranges named after the call site around every launch (with
+// AFQMC_ROCTX=1 in the environment), so that `rocprofv3 --kernel-rename`
+// reports the walker-step phases rather than template instantiations.
+struct Roctx {
+  int (*push)(const char*) = nullptr;
+  int (*pop)() = nullptr;
+};
+const Roctx& roctx()
+{
+  static const Roctx r = []() {
+    Roctx x;
+    const char* e = std::getenv("AFQMC_ROCTX");
+    if (!e || !*e || *e == '0') return x;
+    void* h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return x;
+    x.push = (int (*)(const char*))dlsym(h, "roctxRangePushA");
+    x.pop = (int (*)())dlsym(h, "roctxRangePop");
+    if (!x.push || !x.pop) x.push = nullptr, x.pop = nullptr;
+    return x;
+  }();
+  return r;
+}
+
 template <class F> int launch(afqmc_ctx* c, const char* name, F&& f)
 {
+  const Roctx& rx = roctx();
+  if (rx.push) rx.push(c->tuning ? (std::string("tune/") + name).c_str() : name);
+  struct Pop {
+    const Roctx& r;
+    ~Pop()
+    {
+      if (r.pop) r.pop();
+    }
+  } pop_{rx};
   hipEvent_t a = nullptr, b = nullptr;
   if (c->prof) {
     a = get_event(c);
@@ -1652,8 +1686,11 @@ int afqmc_tune(afqmc_ctx* c, double dt)
 {
   NEED_SYSTEM(c);
   if (!c->xstage || c->nstage < 1) return fail(c, AFQMC_ERR_STATE, "stage fields before tuning");
+  // tune on walkers at the trial (a representative, well-conditioned state)
+  int r = afqmc_init_walkers(c);
+  if (r != AFQMC_OK) return r;
   c->tuning = true;
-  int r = substep_body(c, c->xstage, 1, dt);
+  r = substep_body(c, c->xstage, 1, dt);
   if (r == AFQMC_OK) r = do_mdm(c);
   if (r == AFQMC_OK) r = do_energy(c, nullptr);
   c->tuning = false;
