@@ -166,7 +166,7 @@ int afqmc_energy(afqmc_ctx* ctx, double* eav_out);
 /* Complex GEMM on the GPU with ma::product's row-major meaning
  * (src/Numerics/ma_operations.hpp:94-107 -> BLAS::gemm,
  * src/Numerics/OhmmsBlas.h:270-278): C[m][n] = alpha*op(A)*op(B) + beta*C,
- * op in {'N','T'}, host arrays.  precision: 0 FP64, 1 FP32, 2 split-BF16
+ * op(A) in {'N','T'}, op(B) in {'N','T','C' (B^H),'R' (conj(B))}, host arrays.  precision: 0 FP64, 1 FP32, 2 split-BF16
  * (the arithmetic of the modes above); bits 8..15 of `precision` may force
  * one tile candidate (1-based) instead of the size heuristic. */
 int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int k, const double* alpha, const double* A,

@@ -89,10 +89,9 @@ struct afqmc_ctx {
   void* vHS = nullptr;    // CSR: zd [M*M][nw]; dense: [nw][M*M] (zd / zf)
   zd* vHSw = nullptr;     // CSR: [nw][M*M]
   zd* hybridW = nullptr;
-  void* S = nullptr;      // propagation scratch [nw][2][M][N] x 4 (zd / zf)
+  void* S = nullptr;      // propagation scratch [nw][2][M][N] x 3: S = P W and the Horner terms
   void* Ta = nullptr;
   void* Tb = nullptr;
-  void* Sacc = nullptr;
   double* eshift = nullptr;
   int* flags = nullptr;   // [0] singular count, [1] non-finite fields
   double* xstage = nullptr;
@@ -660,30 +659,33 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     a.C = c->S;
     RET((zgemm<CP, PT, double, ST, true, false>(c, "prop_pw", a, 2 * nw)));
   }
+  // exp(iV) S to sixth order (vHS.hpp:61-91) by Horner's rule:
+  // U_7 = S, U_k = S + (i/k) V U_{k+1}, E6(V) S = U_1; each step is one GEMM
+  // whose epilogue adds S.
   const void* prev = c->S;
   void* bufs[2] = {c->Ta, c->Tb};
-  for (int n = 1; n <= 6; n++) {
+  for (int k = 6; k >= 1; k--) {
     ZGemmArgs a = base();
     a.A = V;
     a.sa1 = MM;
     a.sa2 = 0;
     a.B = prev;
-    a.C = (n < 6) ? bufs[n & 1] : nullptr;
+    a.C = bufs[k & 1];
     a.ar = 0.0;
-    a.ai = 1.0 / (double)n;
-    a.D = c->Sacc;
-    a.F = (n == 1) ? c->S : c->Sacc;
-    if (n == 1) RET((zgemm<CP, VT, ST, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
+    a.ai = 1.0 / (double)k;
+    a.F = c->S;
+    a.cf = 1;
+    if (k == 6) RET((zgemm<CP, VT, ST, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
     else RET((zgemm<CP, VT, TT, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
-    prev = bufs[n & 1];
+    prev = bufs[k & 1];
   }
-  // W = P * S
+  // W = P * E6(V) S
   {
     ZGemmArgs a = base();
     a.A = P;
-    a.B = c->Sacc;
+    a.B = prev;
     a.C = c->W;
-    RET((zgemm<CP, PT, ST, double, true, false>(c, "prop_ps", a, 2 * nw)));
+    RET((zgemm<CP, PT, TT, double, true, false>(c, "prop_ps", a, 2 * nw)));
   }
   return AFQMC_OK;
 }
@@ -1175,7 +1177,6 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   chk(dalloc_bytes(c, &c->S, nw * 2 * MN * zs));
   chk(dalloc_bytes(c, &c->Ta, nw * 2 * MN * zs));
   chk(dalloc_bytes(c, &c->Tb, nw * 2 * MN * zs));
-  chk(dalloc_bytes(c, &c->Sacc, nw * 2 * MN * zs));
   chk(dalloc(c, &c->eshift, 1));
   chk(dalloc(c, &c->flags, 4));
   chk(dalloc(c, &c->e_direct, nw));
@@ -1765,10 +1766,13 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
   if (!alpha || !beta || !A || !B || !Cm || m < 0 || n < 0 || k < 0) return AFQMC_ERR_ARG;
   const int tile = (precision >> 8) & 0xff;
   precision &= 0xff;
-  if ((opA != 'N' && opA != 'T') || (opB != 'N' && opB != 'T') || precision < 0 || precision > 2)
+  if ((opA != 'N' && opA != 'T') || (opB != 'N' && opB != 'T' && opB != 'C' && opB != 'R') || precision < 0 ||
+      precision > 2)
     return AFQMC_ERR_ARG;
+  // opB 'C': B^H (B stored n x k), 'R': conj(B) (B stored k x n)
+  const bool bn = (opB == 'N' || opB == 'R');
   const long ar = (opA == 'N') ? m : k, ac = (opA == 'N') ? k : m;
-  const long br = (opB == 'N') ? k : n, bc = (opB == 'N') ? n : k;
+  const long br = bn ? k : n, bc = bn ? n : k;
   if (lda < ac || ldb < bc || ldc < n) return AFQMC_ERR_SHAPE;
   if (m == 0 || n == 0) return AFQMC_OK;
   afqmc_ctx tmp;
@@ -1788,7 +1792,8 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
     a.A = dA;
     a.B = dB;
     a.C = dC;
-    const bool ak = (opA == 'N'), bk = (opB == 'T');
+    const bool ak = (opA == 'N'), bk = !bn;
+    a.conjb = (opB == 'C' || opB == 'R') ? 1 : 0;
     a.sam = ak ? lda : 1;
     a.sak = ak ? 1 : lda;
     a.sbn = bk ? ldb : 1;

@@ -51,6 +51,8 @@ struct ZGemmArgs {
   int ktri;
   // use conj(B) in place of B
   int conjb;
+  // C(m,n) = F(m,n) + alpha*AB (F with C's strides, D unused)
+  int cf;
 };
 
 template <int CP> struct LdsT;
@@ -61,10 +63,6 @@ template <> struct LdsT<CP_F64> {
 template <> struct LdsT<CP_F32> {
   using T = zf;
   static constexpr int PAD = 2;
-};
-template <> struct LdsT<CP_BF16X3> {
-  using T = uint32_t;  // re | im << 16 (bf16 bits)
-  static constexpr int PAD = 4;
 };
 
 template <class Z> __device__ inline void zero_z(Z& z)
@@ -92,13 +90,6 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   static_assert(CP == CP_BF16X3 || BKC % 4 == 0, "k-tile must be a multiple of 4 complex");
   static_assert(CP == CP_BF16X3 || (!std::is_same<AT, bfs>::value && !std::is_same<BT, bfs>::value),
                 "split-BF16 operands need the BF16 path");
-  using L = LdsT<CP>;
-  using LT = typename L::T;
-  constexpr int LDK = BKC + L::PAD;
-  constexpr int NPLANE = (CP == CP_BF16X3) ? 2 : 1;
-  // two LDS stages (ping-pong): the next k-tile is stored while others compute
-  __shared__ __align__(16) LT sA[2][NPLANE][BM * LDK];
-  __shared__ __align__(16) LT sB[2][NPLANE][BNC * LDK];
   using AZ = typename ztype<AT>::type;
   using BZ = typename ztype<BT>::type;
   using ACC = typename std::conditional<CP == CP_F64, doublex4, floatx4>::type;
@@ -151,6 +142,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
 #pragma unroll
       for (int q = 0; q < 4; q++) acc[i][j][q] = 0;
 
+  if constexpr (CP != CP_BF16X3) {
+  using L = LdsT<CP>;
+  using LT = typename L::T;
+  constexpr int LDK = BKC + L::PAD;
+  // two LDS stages (ping-pong): the next k-tile is stored while others compute
+  __shared__ __align__(16) LT sA[2][BM * LDK];
+  __shared__ __align__(16) LT sB[2][BNC * LDK];
   AZ ra[EA];
   BZ rb[EB];
   auto gload = [&](int k0) {
@@ -181,16 +179,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int row, kc;
       if (AK) { row = e / BKC; kc = e % BKC; } else { kc = e / BM; row = e % BM; }
       if constexpr (CP == CP_F64) {
-        sA[st][0][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
-      } else if constexpr (CP == CP_F32) {
-        sA[st][0][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
-      } else if constexpr (std::is_same<AZ, zb>::value) {
-        sA[st][0][row * LDK + kc] = ra[i].hi;
-        sA[st][1][row * LDK + kc] = ra[i].lo;
+        sA[st][row * LDK + kc] = zd{(double)ra[i].re, (double)ra[i].im};
       } else {
-        const zb v = split_z((float)ra[i].re, (float)ra[i].im);
-        sA[st][0][row * LDK + kc] = v.hi;
-        sA[st][1][row * LDK + kc] = v.lo;
+        sA[st][row * LDK + kc] = zf{(float)ra[i].re, (float)ra[i].im};
       }
     }
 #pragma unroll
@@ -201,19 +192,10 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       if (BK) { col = e / BKC; kc = e % BKC; } else { kc = e / BNC; col = e % BNC; }
       if constexpr (CP == CP_F64) {
         const double bi = (double)rb[i].im;
-        sB[st][0][col * LDK + kc] = zd{(double)rb[i].re, p.conjb ? -bi : bi};
-      } else if constexpr (CP == CP_F32) {
-        const float bi = (float)rb[i].im;
-        sB[st][0][col * LDK + kc] = zf{(float)rb[i].re, p.conjb ? -bi : bi};
-      } else if constexpr (std::is_same<BZ, zb>::value) {
-        const uint32_t flip = p.conjb ? 0x80000000u : 0u;  // sign of the imaginary halves
-        sB[st][0][col * LDK + kc] = rb[i].hi ^ flip;
-        sB[st][1][col * LDK + kc] = rb[i].lo ^ flip;
+        sB[st][col * LDK + kc] = zd{(double)rb[i].re, p.conjb ? -bi : bi};
       } else {
         const float bi = (float)rb[i].im;
-        const zb v = split_z((float)rb[i].re, p.conjb ? -bi : bi);
-        sB[st][0][col * LDK + kc] = v.hi;
-        sB[st][1][col * LDK + kc] = v.lo;
+        sB[st][col * LDK + kc] = zf{(float)rb[i].re, p.conjb ? -bi : bi};
       }
     }
   };
@@ -227,15 +209,15 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BKC);
-    if constexpr (CP == CP_F64 || CP == CP_F32) {
+    {
       using RT = typename std::conditional<CP == CP_F64, double, float>::type;
 #pragma unroll
       for (int kq = 0; kq < BKC / 4; kq++) {
         LT a[TM], bv[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) a[i] = sA[cur][0][(arow0 + i * 16) * LDK + kq * 4 + g];
+        for (int i = 0; i < TM; i++) a[i] = sA[cur][(arow0 + i * 16) * LDK + kq * 4 + g];
 #pragma unroll
-        for (int j = 0; j < TN; j++) bv[j] = sB[cur][0][(bcol0 + j * 8) * LDK + kq * 4 + g];
+        for (int j = 0; j < TN; j++) bv[j] = sB[cur][(bcol0 + j * 8) * LDK + kq * 4 + g];
 #pragma unroll
         for (int j = 0; j < TN; j++) {
           RT b0 = s ? bv[j].im : bv[j].re;
@@ -252,45 +234,6 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
           }
         }
       }
-    } else {
-#pragma unroll
-      for (int kq = 0; kq < BKC / 16; kq++) {
-        uint4 ah[TM], al[TM], bh[TN], bl[TN];
-#pragma unroll
-        for (int i = 0; i < TM; i++) {
-          ah[i] = *(const uint4*)&sA[cur][0][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
-          al[i] = *(const uint4*)&sA[cur][1][(arow0 + i * 16) * LDK + kq * 16 + 4 * g];
-        }
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-          uint4 h = *(const uint4*)&sB[cur][0][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
-          uint4 l = *(const uint4*)&sB[cur][1][(bcol0 + j * 8) * LDK + kq * 16 + 4 * g];
-          if (s == 0) {
-            h.x ^= 0x80000000u; h.y ^= 0x80000000u; h.z ^= 0x80000000u; h.w ^= 0x80000000u;
-            l.x ^= 0x80000000u; l.y ^= 0x80000000u; l.z ^= 0x80000000u; l.w ^= 0x80000000u;
-          } else {
-            h.x = (h.x >> 16) | (h.x << 16); h.y = (h.y >> 16) | (h.y << 16);
-            h.z = (h.z >> 16) | (h.z << 16); h.w = (h.w >> 16) | (h.w << 16);
-            l.x = (l.x >> 16) | (l.x << 16); l.y = (l.y >> 16) | (l.y << 16);
-            l.z = (l.z >> 16) | (l.z << 16); l.w = (l.w >> 16) | (l.w << 16);
-          }
-          bh[j] = h;
-          bl[j] = l;
-        }
-#pragma unroll
-        for (int i = 0; i < TM; i++) {
-          bf16x8 xah = __builtin_bit_cast(bf16x8, ah[i]);
-          bf16x8 xal = __builtin_bit_cast(bf16x8, al[i]);
-#pragma unroll
-          for (int j = 0; j < TN; j++) {
-            bf16x8 xbh = __builtin_bit_cast(bf16x8, bh[j]);
-            bf16x8 xbl = __builtin_bit_cast(bf16x8, bl[j]);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xah, xbh, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xah, xbl, acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xal, xbh, acc[i][j], 0, 0, 0);
-          }
-        }
-      }
     }
     // stage cur^1 was last read in iteration kt-1, which every thread has
     // finished (barrier below), so it can be refilled now
@@ -298,36 +241,219 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
     __syncthreads();
   }
 
-  // epilogue: lanes (2c, 2c+1) hold Re and Im of complex column c
-  const bool has_beta = (p.br != 0.0 || p.bi != 0.0);
+  } else {
+  // Split-BF16: the LDS tiles keep the interleaved (hi, lo) words of each
+  // complex number, rows of LDZ = BKC + 2 entries (144 B at BKC = 16), so
+  // that the 16-byte stores of k pairs and the 2 x 16-byte fragment reads
+  // of 4 consecutive k are both free of bank conflicts.
+  constexpr int BKP = BKC / 2;
+  constexpr int LDZ = BKC + 2;
+  constexpr int PA = (BM * BKP + NT - 1) / NT;
+  constexpr int PB = (BNC * BKP + NT - 1) / NT;
+  __shared__ __align__(16) uint2 zA[2][BM * LDZ];
+  __shared__ __align__(16) uint2 zB[2][BNC * LDZ];
+  // global -> register prefetch of the next k-tile while the MFMAs run.
+  // Each thread's operand pointers are fixed over the k loop; tiles that lie
+  // inside the matrix in m / n load without per-element checks except in the
+  // k-tile that crosses kend.
+  static_assert((BM * BKP) % NT == 0 && (BNC * BKP) % NT == 0, "tile must split evenly over the threads");
+  AZ ra[PA][2];
+  BZ rb[PB][2];
+  const AZ* pa[PA];
+  const BZ* pb[PB];
+  int ka[PA], kb[PB];
+  bool va[PA], vb[PB];
 #pragma unroll
-  for (int i = 0; i < TM; i++)
+  for (int i = 0; i < PA; i++) {
+    const int e = tid + i * NT;
+    int row, kp;
+    if (AK) { row = e / BKP; kp = e % BKP; } else { kp = e / BM; row = e % BM; }
+    const int gm = m0 + row;
+    va[i] = gm < p.m;
+    ka[i] = 2 * kp;
+    pa[i] = A + (long)(va[i] ? gm : 0) * p.sam + (long)ka[i] * p.sak;
+  }
+#pragma unroll
+  for (int i = 0; i < PB; i++) {
+    const int e = tid + i * NT;
+    int col, kp;
+    if (BK) { col = e / BKP; kp = e % BKP; } else { kp = e / BNC; col = e % BNC; }
+    const int gn = n0 + col;
+    vb[i] = gn < p.n;
+    kb[i] = 2 * kp;
+    pb[i] = B + (long)(vb[i] ? gn : 0) * p.sbn + (long)kb[i] * p.sbk;
+  }
+  const bool interior = (m0 + BM <= p.m) && (n0 + BNC <= p.n);
+  auto gload = [&](int k0) {
+    const long oa = (long)k0 * p.sak, ob = (long)k0 * p.sbk;
+    if (interior && k0 + BKC <= kend) {
+#pragma unroll
+      for (int i = 0; i < PA; i++) {
+        ra[i][0] = pa[i][oa];
+        ra[i][1] = pa[i][oa + p.sak];
+      }
+#pragma unroll
+      for (int i = 0; i < PB; i++) {
+        rb[i][0] = pb[i][ob];
+        rb[i][1] = pb[i][ob + p.sbk];
+      }
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < PA; i++) {
+      const int gk = k0 + ka[i];
+      if (va[i] && gk < kend) ra[i][0] = pa[i][oa];
+      else zero_z(ra[i][0]);
+      if (va[i] && gk + 1 < kend) ra[i][1] = pa[i][oa + p.sak];
+      else zero_z(ra[i][1]);
+    }
+#pragma unroll
+    for (int i = 0; i < PB; i++) {
+      const int gk = k0 + kb[i];
+      if (vb[i] && gk < kend) rb[i][0] = pb[i][ob];
+      else zero_z(rb[i][0]);
+      if (vb[i] && gk + 1 < kend) rb[i][1] = pb[i][ob + p.sbk];
+      else zero_z(rb[i][1]);
+    }
+  };
+  auto tozb = [](auto v, bool conj) -> zb {
+    if constexpr (std::is_same<decltype(v), zb>::value) {
+      const uint32_t flip = conj ? 0x80000000u : 0u;  // sign of the imaginary halves
+      return zb{v.hi ^ flip, v.lo ^ flip};
+    } else {
+      const float im = (float)v.im;
+      return split_z((float)v.re, conj ? -im : im);
+    }
+  };
+  auto sstore = [&](int st) {
+#pragma unroll
+    for (int i = 0; i < PA; i++) {
+      const int e = tid + i * NT;
+      if (e >= BM * BKP) break;
+      int row, kp;
+      if (AK) { row = e / BKP; kp = e % BKP; } else { kp = e / BM; row = e % BM; }
+      const zb v0 = tozb(ra[i][0], false), v1 = tozb(ra[i][1], false);
+      *(uint4*)&zA[st][row * LDZ + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
+    }
+#pragma unroll
+    for (int i = 0; i < PB; i++) {
+      const int e = tid + i * NT;
+      if (e >= BNC * BKP) break;
+      int col, kp;
+      if (BK) { col = e / BKP; kp = e % BKP; } else { kp = e / BNC; col = e % BNC; }
+      const zb v0 = tozb(rb[i][0], p.conjb != 0), v1 = tozb(rb[i][1], p.conjb != 0);
+      *(uint4*)&zB[st][col * LDZ + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
+    }
+  };
+
+  const int arow0 = wm * WTM + r;           // + bi*16
+  const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
+  const uint32_t brot = s ? 16u : 0u, bneg = s ? 0u : 0x80000000u;
+  auto compute = [&](int cur) {
+#pragma unroll
+    for (int kq = 0; kq < BKC / 16; kq++) {
+      uint4 ah[TM], al[TM], bh[TN], bl[TN];
+#pragma unroll
+      for (int i = 0; i < TM; i++) {
+        const uint4* pa = (const uint4*)&zA[cur][(arow0 + i * 16) * LDZ + kq * 16 + 4 * g];
+        const uint4 u0 = pa[0], u1 = pa[1];
+        ah[i] = make_uint4(u0.x, u0.z, u1.x, u1.z);
+        al[i] = make_uint4(u0.y, u0.w, u1.y, u1.w);
+      }
+#pragma unroll
+      for (int j = 0; j < TN; j++) {
+        const uint4* pb = (const uint4*)&zB[cur][(bcol0 + j * 8) * LDZ + kq * 16 + 4 * g];
+        const uint4 u0 = pb[0], u1 = pb[1];
+        // lanes s = 0 (real output column) take (b_re, -b_im), lanes s = 1
+        // (imaginary column) take (b_im, b_re): one xor and one rotate per
+        // word, no branch
+        auto ex = [&](uint32_t w) { return __builtin_amdgcn_alignbit(w, w, brot) ^ bneg; };
+        bh[j] = make_uint4(ex(u0.x), ex(u0.z), ex(u1.x), ex(u1.z));
+        bl[j] = make_uint4(ex(u0.y), ex(u0.w), ex(u1.y), ex(u1.w));
+      }
+      // the three products of each accumulator are issued TM*TN MFMAs apart,
+      // so consecutive MFMAs never wait on each other's result
+#pragma unroll
+      for (int t = 0; t < 3; t++)
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+          for (int j = 0; j < TN; j++) {
+            const bf16x8 xa = __builtin_bit_cast(bf16x8, t == 2 ? al[i] : ah[i]);
+            const bf16x8 xb = __builtin_bit_cast(bf16x8, t == 1 ? bl[j] : bh[j]);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, xb, acc[i][j], 0, 0, 0);
+          }
+    }
+  };
+  const int nk = (kend + BKC - 1) / BKC;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; kt++) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BKC);
+    compute(cur);
+    // stage cur^1 was last read in iteration kt-1, before its barrier
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+  }
+
+  // epilogue: lanes (2c, 2c+1) hold Re and Im of complex column c.  The
+  // values it reads (F, and C when beta != 0) are loaded for the whole tile
+  // first: D may alias F, so loads issued between the stores could not
+  // overlap them.
+  const bool has_beta = (p.br != 0.0 || p.bi != 0.0);
+  // FP64 16x16x4 interleaves rows (row = 4*reg + lane/16); the FP32 and BF16
+  // 16x16 forms keep 4 consecutive rows per lane group.
+  const int rbase = m0 + wm * WTM + ((CP == CP_F64) ? g : 4 * g);
+  constexpr int RQ = (CP == CP_F64) ? 4 : 1;  // row step of the accumulator register
+  const int cbase = n0 + (wn * WTN + r) / 2;
+  // one accumulator row block at a time: its F (and C for beta) values are
+  // loaded first, then the block is written
+#pragma unroll
+  for (int i = 0; i < TM; i++) {
+    DT fv[TN][4];
+    CT cv[TN][4];
 #pragma unroll
     for (int j = 0; j < TN; j++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        double v = (double)acc[i][j][q];
-        double pv = __shfl_xor(v, 1);
-        double re = s ? pv : v, im = s ? v : pv;
-        // FP64 16x16x4 interleaves rows (row = 4*reg + lane/16); the FP32 and
-        // BF16 16x16 forms keep 4 consecutive rows per lane group.
-        int row = m0 + wm * WTM + i * 16 + ((CP == CP_F64) ? (4 * q + g) : (4 * g + q));
-        int col = n0 + (wn * WTN + j * 16 + r) / 2;
+        const int row = rbase + i * 16 + RQ * q, col = cbase + j * 8;
+        const bool in = row < p.m && col < p.n;
+        const long off = 2 * ((long)row * p.scm + (long)col * p.scn);
+        fv[j][q] = ((D || p.cf) && F && in) ? F[off + s] : (DT)0;
+        if constexpr (!OSPLIT) cv[j][q] = (C && has_beta && in) ? C[off + s] : (CT)0;
+      }
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int row = rbase + i * 16 + RQ * q, col = cbase + j * 8;
+        const double v = (double)acc[i][j][q];
+        const double pv = __shfl_xor(v, 1);
+        const double re = s ? pv : v, im = s ? v : pv;
+        const double o = s ? (p.ar * im + p.ai * re) : (p.ar * re - p.ai * im);  // alpha * AB
+        double oc = o;                                                              // + beta * C
+        if constexpr (!OSPLIT) {
+          if (has_beta) {
+            const double cs = (double)cv[j][q], cp = __shfl_xor(cs, 1);
+            const double cr = s ? cp : cs, ci = s ? cs : cp;
+            oc += s ? (p.br * ci + p.bi * cr) : (p.br * cr - p.bi * ci);
+          }
+        }
+        const double ofc = p.cf ? (double)fv[j][q] + o : o;  // C = F + alpha*AB
+        const double oi = __shfl_xor(ofc, 1);
         if (row < p.m && col < p.n) {
-          long off = 2 * ((long)row * p.scm + (long)col * p.scn);
-          double o = s ? (p.ar * im + p.ai * re) : (p.ar * re - p.ai * im);
-          if (D) D[off + s] = (DT)((F ? (double)F[off + s] : 0.0) + o);
+          const long off = 2 * ((long)row * p.scm + (long)col * p.scn);
+          if (D) D[off + s] = (DT)((double)fv[j][q] + o);
           if constexpr (OSPLIT) {
-            // lane s=0 writes the whole split complex (alpha applied, no beta)
-            const double oi = __shfl_xor(o, 1);
-            if (CS && s == 0) CS[off / 2] = split_z((float)o, (float)oi);
+            // lane s=0 writes the whole split complex (no beta)
+            if (CS && s == 0) CS[off / 2] = split_z((float)ofc, (float)oi);
           } else if (C) {
-            if (has_beta) {
-              double cr = (double)C[off], ci = (double)C[off + 1];
-              o += s ? (p.br * ci + p.bi * cr) : (p.br * cr - p.bi * ci);
-            }
-            C[off + s] = (CT)o;
+            C[off + s] = (CT)(p.cf ? oc + (double)fv[j][q] : oc);
           }
         }
       }
+  }
 }
