@@ -72,6 +72,9 @@ def main():
     ap.add_argument("--cpu", type=int, default=1, help="time the oracle on a walker sample (0 to skip)")
     ap.add_argument("--cpu-walkers", type=int, default=6)
     ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--energy-level", type=int, default=3,
+                    help="mixed modes: 0 energy in the contraction precision, 1 FP64 G/one-body/Coulomb, "
+                         "2 and FP32 exchange, 3 and FP64 exchange")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
@@ -137,7 +140,7 @@ def main():
     def run_mode(mode, profile, state=None):
         """Time `args.steps` substeps in `mode`; then run the accuracy probes
         from `state`, or from this run's own final walkers (returned)."""
-        g = m.AfqmcSys(s, nw, mode)
+        g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level)
         g.stage_fields(x)
         if args.tune:
             tt = time.perf_counter()
@@ -259,7 +262,7 @@ def main():
         "dtype": MODE_DTYPE[args.mode],
         "data": "synthetic (seeded Hamiltonian and auxiliary fields)",
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
-                   "mode": args.mode, "northo": northo,
+                   "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
                    "energy": "none" if args.no_energy else f"once per {args.steps} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -505,8 +505,11 @@ int do_mdm(afqmc_ctx* c)
   a.scn = 1;
   a.sc1 = 2L * N * M;
   a.sc2 = (long)N * M;
-  if (c->bsplit) return zgemm<CP_F64, double, double, bfs, true, true>(c, "gc_gemm", a, 2 * nw);
-  if (c->mixed) return zgemm<CP_F64, double, double, float, true, true>(c, "gc_gemm", a, 2 * nw);
+  // mixed modes: G only feeds the force bias, which contracts it in the
+  // contraction precision, so it is formed in that precision as well (the
+  // energy rebuilds an FP64 G from the FP64 inverses)
+  if (c->bsplit) return zgemm<CP_BF16X3, double, double, bfs, true, true>(c, "gc_gemm", a, 2 * nw);
+  if (c->mixed) return zgemm<CP_F32, double, double, float, true, true>(c, "gc_gemm", a, 2 * nw);
   return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
 }
 
