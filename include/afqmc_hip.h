@@ -68,7 +68,8 @@ typedef enum {
    * exchange), E_L = their sum; read-only.  With Vakbl the whole two-body
    * energy sum G.(Vakbl G)/2 (energy.hpp:93-95) is the second term and the
    * third is zero. */
-  AFQMC_BUF_EPARTS = 7
+  AFQMC_BUF_EPARTS = 7,
+  AFQMC_BUF_GFULL = 8    /* full G [2*M*M][nwalk] (non-transposed bias only) */
 } afqmc_buffer;
 
 /* ---- context ------------------------------------------------------------ */
@@ -101,7 +102,12 @@ typedef enum {
    * matrices too ill-conditioned for that route (and N > 128) use the blocked
    * Householder kernel.  1: always the blocked Householder kernel.  May be
    * changed at any time. */
-  AFQMC_OPT_ORTHO_HOUSEHOLDER = 2
+  AFQMC_OPT_ORTHO_HOUSEHOLDER = 2,
+  /* the driver's -t flag (src/miniapps/miniafqmc_base.cpp:256-276).  1
+   * (default): compact G and the half-rotated SpvnT for the bias.  0: full G
+   * and T(Spvn) (AFQMC_FP64_CSR only; set before afqmc_upload_system, which
+   * then also builds the transposed CSR of Spvn). */
+  AFQMC_OPT_TRANSPOSED = 3
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 
@@ -137,6 +143,14 @@ int afqmc_copy(afqmc_ctx* ctx, int buffer, int to_device, double* host);
 int afqmc_mixed_density_matrix(afqmc_ctx* ctx);
 /* base::get_vbias(SpvnT, Gc, vbias, transposed=true) (src/AFQMC/vbias.hpp:45-83) */
 int afqmc_vbias(afqmc_ctx* ctx);
+/* Replaces afqmc_sys::calculate_mixed_density_matrix(W, W_data, G, false)
+ * (src/AFQMC/afqmc_sys.hpp:95-116 with mixed_density_matrix.hpp:93-101):
+ * full G [2*M*M][nwalk] = conj(A) T1^-1 W^T per spin, and the overlaps.
+ * Requires AFQMC_OPT_TRANSPOSED = 0. */
+int afqmc_mixed_density_matrix_full(afqmc_ctx* ctx);
+/* Replaces base::get_vbias(Spvn, G, vbias, false) (src/AFQMC/vbias.hpp:65-82):
+ * vbias = T(Spvn) G_alpha + T(Spvn) G_beta from the full G. */
+int afqmc_vbias_full(afqmc_ctx* ctx);
 /* X/hybrid-weight loop (src/miniapps/miniafqmc_base.cpp:281-289): x holds the
  * real N(0,1) draws [K][nwalk] (NULL: use the fields staged for substep
  * `substep` by afqmc_stage_fields).  X = x + i vbias; hybridW. */

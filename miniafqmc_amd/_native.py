@@ -24,14 +24,15 @@ STATUS = {
 
 MODES = {"fp64_csr": 0, "fp64_dense": 1, "mixed_fp32": 2, "mixed_bf16": 3}
 
-BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS = range(8)
-OPT_DENSE_EXCHANGE, OPT_ENERGY_PRECISION, OPT_ORTHO_HOUSEHOLDER = 0, 1, 2
+BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS, BUF_GFULL = range(9)
+OPT_DENSE_EXCHANGE, OPT_ENERGY_PRECISION, OPT_ORTHO_HOUSEHOLDER, OPT_TRANSPOSED = 0, 1, 2, 3
 
 # every symbol include/afqmc_hip.h declares
 EXPORTED = [
     "afqmc_ctx_create", "afqmc_ctx_destroy", "afqmc_ctx_error", "afqmc_sync", "afqmc_set_option",
     "afqmc_upload_system", "afqmc_get_spvnT", "afqmc_init_walkers", "afqmc_copy",
-    "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_fields", "afqmc_vhs",
+    "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_mixed_density_matrix_full", "afqmc_vbias_full",
+    "afqmc_fields", "afqmc_vhs",
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
     "afqmc_weights", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
     "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
@@ -101,6 +102,8 @@ def _load():
         "afqmc_copy": (I, [P, I, I, D]),
         "afqmc_mixed_density_matrix": (I, [P]),
         "afqmc_vbias": (I, [P]),
+        "afqmc_mixed_density_matrix_full": (I, [P]),
+        "afqmc_vbias_full": (I, [P]),
         "afqmc_fields": (I, [P, D, L]),
         "afqmc_vhs": (I, [P]),
         "afqmc_propagate": (I, [P]),

@@ -23,7 +23,8 @@ def _dp(a):
 
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
-                 use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1):
+                 use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1,
+                 transposed: bool = True):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
@@ -32,6 +33,7 @@ class AfqmcSys:
         self.ctx = h
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DENSE_EXCHANGE, 1.0 if dense_exchange else 0.0))
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ENERGY_PRECISION, float(energy_precision)))
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_TRANSPOSED, 1.0 if transposed else 0.0))
         ta = np.ascontiguousarray(system.trial_alpha)
         tb = np.ascontiguousarray(system.trial_beta)
         sp = system.spvn.view()
@@ -68,6 +70,7 @@ class AfqmcSys:
         nat.BUF_VHS: lambda s: (s.M * s.M, s.nwalk),
         nat.BUF_HYBRIDW: lambda s: (s.nwalk,),
         nat.BUF_EPARTS: lambda s: (s.nwalk, 3),
+        nat.BUF_GFULL: lambda s: (2 * s.M * s.M, s.nwalk),
     }
 
     def get(self, buf):
@@ -114,6 +117,16 @@ class AfqmcSys:
         xa = np.ascontiguousarray(x, dtype=np.float64)
         assert xa.shape == (self.K, self.nwalk)
         self._check(nat.lib.afqmc_fields(self.ctx, _dp(xa), 0))
+
+    def calculate_mixed_density_matrix_full(self):
+        """calculate_mixed_density_matrix(W, W_data, G, false) (mixed_density_matrix.hpp:93-101): G [2M*M][nwalk]."""
+        self._check(nat.lib.afqmc_mixed_density_matrix_full(self.ctx))
+        return self.get(nat.BUF_GFULL)
+
+    def get_vbias_full(self):
+        """base::get_vbias(Spvn, G, vbias, false) (vbias.hpp:65-82)."""
+        self._check(nat.lib.afqmc_vbias_full(self.ctx))
+        return self.get(nat.BUF_VBIAS)
 
     def get_vHS(self):
         """base::get_vHS(Spvn, X, vHS) (vHS.hpp:43-56)."""

@@ -64,6 +64,13 @@ struct afqmc_ctx {
   int* spt_col = nullptr;
   zd* spt_val = nullptr;
   long long spt_nnz = 0;
+  // non-transposed bias (miniafqmc_base.cpp:267-276): T(Spvn) as CSR [K][M*M]
+  // and the full density matrix G [2*M*M][nw]
+  bool transposed = true;
+  int* sptt_rp = nullptr;
+  int* sptt_col = nullptr;
+  zd* sptt_val = nullptr;
+  zd* Gfull = nullptr;
   int* v_rp = nullptr;
   int* v_col = nullptr;
   zd* v_val = nullptr;
@@ -546,6 +553,56 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, 
     a.br = sp == 0 ? 0.0 : 1.0;
     RET((zgemm<CP, GT, LT_, double, true, true>(c, name, a, 1)));
   }
+  return AFQMC_OK;
+}
+
+// afqmc_sys::calculate_mixed_density_matrix(W, W_data, G, false)
+// (mixed_density_matrix.hpp:93-101): G_s = conj(A_s) Gc_s per walker and
+// spin, [2*M*M][nw] walker-last.  FP64_CSR mode.
+int do_mdm_full(afqmc_ctx* c)
+{
+  const int M = c->M, N = c->N, nw = c->nw;
+  const long MM = (long)M * M, MN = (long)M * N;
+  if (!c->Gfull) RET(dalloc(c, &c->Gfull, 2 * MM * nw));
+  RET(do_mdm(c));
+  ZGemmArgs a = gargs();
+  a.A = c->trial;  // A(i, a) = trialwfn_s[i][a]
+  a.sam = N;
+  a.sak = 1;
+  a.sa1 = 0;
+  a.sa2 = MN;
+  a.B = c->G;  // B(k, a) = Gc[(s N + a) M + k][w]
+  a.sbn = nw;
+  a.sbk = (long)M * nw;
+  a.sb1 = 1;
+  a.sb2 = MN * nw;
+  a.C = c->Gfull;  // C(i, k) = G[(s M + i) M + k][w]
+  a.scm = (long)M * nw;
+  a.scn = nw;
+  a.sc1 = 1;
+  a.sc2 = MM * nw;
+  a.bdiv = 2;
+  a.m = M;
+  a.n = M;
+  a.k = N;
+  return zgemm<CP_F64, double, double, double, true, false>(c, "gfull_gemm", a, 2 * nw);
+}
+
+// base::get_vbias(Spvn, G, vbias, false) (vbias.hpp:65-82):
+// vbias = T(Spvn) G_alpha + T(Spvn) G_beta, as two csrmm of the transposed
+// CSR in the reference's order.
+int do_vbias_full(afqmc_ctx* c)
+{
+  const int nw = c->nw, K = c->K;
+  const long MM = (long)c->M * c->M;
+  zd one{1, 0}, zero{0, 0};
+  const long waves = (long)K * ((nw + 63) / 64);
+  for (int sp = 0; sp < 2; sp++)
+    RET(launch(c, "csrmm_vbias_t", [&]() {
+      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, K, nw, (int)MM, c->sptt_rp,
+                         c->sptt_col, c->sptt_val, (const zd*)c->Gfull + sp * MM * nw, (long)nw, c->vbias, (long)nw,
+                         one, sp ? one : zero);
+    }));
   return AFQMC_OK;
 }
 
@@ -1075,8 +1132,13 @@ int check_flags(afqmc_ctx* c)
 
 int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt)
 {
-  RET(do_mdm(c));
-  RET(do_vbias(c));
+  if (c->transposed) {
+    RET(do_mdm(c));
+    RET(do_vbias(c));
+  } else {
+    RET(do_mdm_full(c));
+    RET(do_vbias_full(c));
+  }
   RET(do_fields(c, xdev));
   RET(do_vhs(c));
   RET(do_propagate(c));
@@ -1239,6 +1301,13 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     }
     return AFQMC_OK;
   }
+  if (option == AFQMC_OPT_TRANSPOSED) {
+    if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
+    if (value == 0.0 && c->mode != AFQMC_FP64_CSR)
+      return fail(c, AFQMC_ERR_UNSUPPORTED, "the non-transposed bias is built for AFQMC_FP64_CSR only");
+    c->transposed = value != 0.0;
+    return AFQMC_OK;
+  }
   if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {
     c->ortho_householder = value != 0.0;
     return AFQMC_OK;
@@ -1331,6 +1400,28 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     c->sp_col = col;
     c->sp_val = val;
     c->sp_nnz = nnz;
+    if (!c->transposed) {
+      // T(Spvn) as CSR [K][M*M] (counting sort by column; rows stay ordered)
+      std::vector<int> trp(K + 1, 0), tcol(nnz);
+      std::vector<zd> tval(nnz);
+      const int r0 = spvn->rowptr[0];
+      for (long q = 0; q < nnz; q++) trp[spvn->col[q] + 1]++;
+      for (long n = 0; n < K; n++) trp[n + 1] += trp[n];
+      std::vector<int> pos(trp.begin(), trp.end() - 1);
+      for (long r = 0; r < MM; r++)
+        for (int q = spvn->rowptr[r] - r0; q < spvn->rowptr[r + 1] - r0; q++) {
+          const int d = pos[spvn->col[q]]++;
+          tcol[d] = (int)r;
+          tval[d] = zd{spvn->val[2 * q], spvn->val[2 * q + 1]};
+        }
+      RET(dalloc(c, &c->sptt_rp, K + 1));
+      RET(dalloc(c, &c->sptt_col, nnz));
+      RET(dalloc(c, &c->sptt_val, nnz));
+      HIPCHK(c, hipMemcpyAsync(c->sptt_rp, trp.data(), sizeof(int) * (K + 1), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->sptt_col, tcol.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->sptt_val, tval.data(), sizeof(zd) * nnz, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+    }
     // SpvnT CSR [K][2NM] with rotate.hpp's cutoff
     int* cnt = nullptr;
     RET(dalloc(c, &cnt, K));
@@ -1525,6 +1616,9 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
       }));
     }
     return AFQMC_OK;
+  case AFQMC_BUF_GFULL:
+    if (!c->Gfull) return fail(c, AFQMC_ERR_STATE, "no full density matrix (non-transposed bias only)");
+    return raw(c->Gfull, sizeof(zd) * 2 * MM * nw);
   case AFQMC_BUF_EPARTS: {
     if (to_device) return fail(c, AFQMC_ERR_ARG, "the energy terms are read-only");
     std::vector<zd> e1(nw), ed(nw), ex(nw);
@@ -1556,6 +1650,21 @@ int afqmc_mixed_density_matrix(afqmc_ctx* c)
   RET(do_mdm(c));
   return check_flags(c);
 }
+int afqmc_mixed_density_matrix_full(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  if (c->transposed) return fail(c, AFQMC_ERR_STATE, "set AFQMC_OPT_TRANSPOSED to 0 before the upload");
+  return do_mdm_full(c);
+}
+
+int afqmc_vbias_full(afqmc_ctx* c)
+{
+  NEED_SYSTEM(c);
+  if (c->transposed) return fail(c, AFQMC_ERR_STATE, "set AFQMC_OPT_TRANSPOSED to 0 before the upload");
+  if (!c->Gfull) return fail(c, AFQMC_ERR_STATE, "form the full density matrix first");
+  return do_vbias_full(c);
+}
+
 int afqmc_vbias(afqmc_ctx* c)
 {
   NEED_SYSTEM(c);

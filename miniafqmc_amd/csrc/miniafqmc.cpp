@@ -31,7 +31,7 @@ void print_help()
   std::printf("-o                Number of substeps between orthogonalization (default: 10)\n");
   std::printf("-f                Input file name (HDF5 input is not built; use --system)\n");
   std::printf("-t                yes|no: half-rotated transposed Cholesky matrix for the bias potential\n");
-  std::printf("                  (only 'yes' is built)\n");
+  std::printf("                  ('no': full G and T(Spvn), fp64_csr mode only)\n");
   std::printf("-v                Verbose output (per-kernel timers)\n");
   std::printf("--dt D            time step (default 0.005)\n");
   std::printf("--mode M          fp64_csr | fp64_dense | mixed_fp32 | mixed_bf16 (default fp64_csr)\n");
@@ -119,10 +119,6 @@ int main(int argc, char** argv)
     std::fprintf(stderr, "Error: HDF5 input (%s) is not built in this version; use --system.\n", init_file.c_str());
     return 1;
   }
-  if (!transposed) {
-    std::fprintf(stderr, "Error: the non-transposed bias potential (-t no) is not built in this version.\n");
-    return 1;
-  }
   const int mode = mode_of(mode_s);
   if (mode < 0 || nsteps < 0 || nsubsteps <= 0 || nwalk <= 0 || northo <= 0 || dt <= 0) {
     print_help();
@@ -171,13 +167,14 @@ int main(int argc, char** argv)
   std::printf("\n  NMO: %d  NAEA: %d  NAEB: %d\n", NMO, NAEA, NAEA);
   std::printf("  Execution details: \n    nsteps: %d\n    nsubsteps: %d\n    nwalk: %d\n    northo: %d\n", nsteps,
               nsubsteps, nwalk, northo);
-  std::printf("    verbose: %s\n    # Chol Vectors: %d\n    transposed Spvn: true\n", verbose ? "true" : "false",
-              nchol);
+  std::printf("    verbose: %s\n    # Chol Vectors: %d\n    transposed Spvn: %s\n", verbose ? "true" : "false",
+              nchol, transposed ? "true" : "false");
   std::printf("    Chol. Matrix Sparsity: %g\n", (double)v.spvn.nnz / ((double)nchol * NMO * NMO));
   std::printf("    Hamiltonian Sparsity: %g\n", v.vakbl.nnz / (NAEA * NAEA * (double)NMO * NMO * 4.0));
   std::printf("    mode: %s  dt: %g\n", mode_s.c_str(), dt);
 
   CHECK(afqmc_ctx_create(NMO, NAEA, nchol, nwalk, mode, &ctx));
+  CHECK(afqmc_set_option(ctx, AFQMC_OPT_TRANSPOSED, transposed ? 1.0 : 0.0));
   CHECK(afqmc_upload_system(ctx, v.trial_alpha, v.trial_beta, v.propg1, v.haj, &v.spvn,
                             v.vakbl.nnz > 0 ? &v.vakbl : nullptr, 1e-6, dt));
   afqmc_synth_destroy(syn);

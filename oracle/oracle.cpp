@@ -877,6 +877,10 @@ int orc_run(int M, int N, int K, int nw, int nsteps, int nsubsteps, int northo, 
   int NAK = 2 * N * M;
   std::vector<cd> vbias((size_t)K * nw), vHS((size_t)M * M * nw), Gc((size_t)NAK * nw), X((size_t)K * nw),
       hybridW(nw);
+  // no SpvnT: the driver's non-transposed branch (miniafqmc_base.cpp:267-276),
+  // full G [2M*M][nw] and vbias = T(Spvn) G_alpha + T(Spvn) G_beta
+  const bool transposed = spt_rp != nullptr;
+  std::vector<cd> Gfull(transposed ? 0 : (size_t)2 * M * M * nw);
   double tsec[11] = {0};
   auto energy = [&]() {
     if (energy_mode == 2) return 0.0;  // timing samples: no energy evaluation
@@ -899,10 +903,22 @@ int orc_run(int M, int N, int K, int nw, int nsteps, int nsubsteps, int northo, 
   for (int step = 0, step_tot = 0; step < nsteps; step++) {
     for (int sub = 0; sub < nsubsteps; sub++, step_tot++) {
       auto t0 = clk::now();
-      int r = sys_mdm(s, nw, W, Wd, Gc.data(), true);
+      int r = 0;
+      if (transposed) {
+        r = sys_mdm(s, nw, W, Wd, Gc.data(), true);
+      } else {
+        r = sys_mdm(s, nw, W, Wd, Gfull.data(), false);
+      }
       if (r && !info) info = r;
       auto t1 = clk::now();
-      csrmm('N', K, nw, NAK, 1.0, CCZ(spt_val), spt_col, spt_rp, spt_rp + 1, Gc.data(), nw, 0.0, vbias.data(), nw);
+      if (transposed) {
+        csrmm('N', K, nw, NAK, 1.0, CCZ(spt_val), spt_col, spt_rp, spt_rp + 1, Gc.data(), nw, 0.0, vbias.data(), nw);
+      } else {
+        const int MM = M * M;
+        csrmm('T', MM, nw, K, 1.0, CCZ(sp_val), sp_col, sp_rp, sp_rp + 1, Gfull.data(), nw, 0.0, vbias.data(), nw);
+        csrmm('T', MM, nw, K, 1.0, CCZ(sp_val), sp_col, sp_rp, sp_rp + 1, Gfull.data() + (size_t)MM * nw, nw, 1.0,
+              vbias.data(), nw);
+      }
       auto t2 = clk::now();
       const double* x = x_all + (size_t)step_tot * K * nw;
       for (size_t i = 0; i < (size_t)K * nw; i++) X[i] = cd(x[i], 0.0);

@@ -74,7 +74,9 @@ long orc_halfrotate_cholesky(int M, int N, int K, const double* trial_a, const d
 int orc_compress(int nrows, long nnz, int* rows, int* cols, double* vals, int* rowptr);
 
 /* full driver loop (miniafqmc_base.cpp:213-349), aux fields from x_all
- * [nsteps*nsubsteps][K][nw].  energy_mode 0: Vakbl CSR, 1: Cholesky form from
+ * [nsteps*nsubsteps][K][nw]; spt_rp == NULL selects the non-transposed
+ * branch (full G and T(Spvn) for the bias, miniafqmc_base.cpp:267-276).
+ * energy_mode 0: Vakbl CSR, 1: Cholesky form from
  * Lt (dense [K][2NM], unscaled), 2: no energy (timing samples).  eav_out has nsteps+1 entries (initial first). */
 int orc_run(int M, int N, int K, int nw, int nsteps, int nsubsteps, int northo, double dt,
             const double* trial_a, const double* trial_b, const double* Propg, const double* haj,

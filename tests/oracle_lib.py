@@ -232,10 +232,12 @@ def compress(nrows, rows, cols, vals):
     return rows, cols, vals, rp
 
 
-def run(sysm, nwalk, nsteps, nsubsteps, northo, x_all, energy_mode=None, spT=None, Lt=None, cutoff=1e-6):
-    """Whole driver loop (miniafqmc_base.cpp:213-349).  Returns (eav[nsteps+1], W, Wd, eshift, timers)."""
+def run(sysm, nwalk, nsteps, nsubsteps, northo, x_all, energy_mode=None, spT=None, Lt=None, cutoff=1e-6,
+        transposed=True):
+    """Whole driver loop (miniafqmc_base.cpp:213-349).  Returns (eav[nsteps+1], W, Wd, eshift, timers).
+    transposed=False takes the driver's non-transposed branch (full G, T(Spvn))."""
     M, N, K = sysm.nmo, sysm.naea, sysm.nchol
-    if spT is None:
+    if spT is None and transposed:
         spT = halfrotate(sysm, cutoff)
     if energy_mode is None:
         energy_mode = 0 if sysm.vakbl is not None else 1
@@ -254,8 +256,9 @@ def run(sysm, nwalk, nsteps, nsubsteps, northo, x_all, energy_mode=None, spT=Non
     dummy_z = np.zeros(1, np.complex128)
     info = lib.orc_run(
         M, N, K, nwalk, nsteps, nsubsteps, northo, C.c_double(sysm.dt), dp(sysm.trial_alpha), dp(sysm.trial_beta),
-        dp(sysm.propg1), dp(sysm.haj), ip(sp.rowptr), ip(sp.col), dp(sp.val), ip(spT.rowptr), ip(spT.col),
-        dp(spT.val), energy_mode, ip(v.rowptr if v is not None else dummy_i), ip(v.col if v is not None else dummy_i),
+        dp(sysm.propg1), dp(sysm.haj), ip(sp.rowptr), ip(sp.col), dp(sp.val),
+        ip(spT.rowptr) if transposed else None, ip(spT.col) if transposed else None,
+        dp(spT.val) if transposed else None, energy_mode, ip(v.rowptr if v is not None else dummy_i), ip(v.col if v is not None else dummy_i),
         dp(v.val if v is not None else dummy_z), dp(Lt if Lt is not None else dummy_z), dp(x_all), dp(W), dp(Wd),
         eav.ctypes.data_as(_D), C.byref(es), tm.ctypes.data_as(_D))
     return eav, W, Wd, es.value, tm, info

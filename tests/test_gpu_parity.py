@@ -284,3 +284,43 @@ def test_mixed_energy_evaluation_error(c1, mode):
         say(f"{mode} energy level {level}: max |dE_L| = {dE.max():.3e} Ha (E_L ~ {eo.real.mean():.3f} Ha)")
         assert dE.max() < tol
         g.close()
+
+
+def test_non_transposed_bias_phases(c1):
+    """Full G (mixed_density_matrix.hpp:93-101) and vbias = T(Spvn) G (vbias.hpp:65-82) vs the oracle."""
+    s, p = c1
+    nw = p["nwalk"]
+    g = m.AfqmcSys(s, nw, "fp64_csr", transposed=False)
+    W = perturbed_walkers(s, nw, seed=17)
+    Wd = np.zeros((nw, 8), complex)
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_WDATA, Wd)
+    say("full G")
+    Gf = g.calculate_mixed_density_matrix_full()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd, compact=False)
+    close(Gf, Go, what="full G")
+    close(g.W_data[:, 2:4], Wd[:, 2:4], what="overlaps")
+    say("vbias from full G")
+    vb = g.get_vbias_full()
+    vbo = orc.get_vbias(s.spvn, Go, transposed=False)
+    close(vb, vbo, what="vbias (non-transposed)")
+    g.close()
+
+
+def test_run_non_transposed_matches_oracle(c1):
+    s, p = c1
+    nw, ns, nsub, northo = p["nwalk"], 6, 2, 3
+    say("run fp64_csr non-transposed N16_M64")
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, ns * nsub)
+    g = m.AfqmcSys(s, nw, "fp64_csr", transposed=False)
+    g.init_walkers()
+    g.stage_fields(x)
+    g.calculate_mixed_density_matrix()
+    e0 = g.calculate_energy()
+    eav = g.run(ns, nsub, northo, s.dt)
+    W, Wd = g.W, g.W_data
+    g.close()
+    eo, Wo, Wdo, eso, tm, info = orc.run(s, nw, ns, nsub, northo, x, transposed=False)
+    close(np.concatenate([[e0], eav]), eo, 1e-9, "Eav trajectory")
+    close(W, Wo, 1e-9, "final walkers")
+    close(Wd[:, :4], Wdo[:, :4], 1e-9, "final W_data")

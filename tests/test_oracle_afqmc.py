@@ -276,3 +276,18 @@ def test_run_composes_the_pieces(small):
     np.testing.assert_allclose(Wdr, Wd, rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(eav[1], e1, rtol=1e-13)
     np.testing.assert_allclose(esr, es, rtol=1e-13)
+
+
+def test_run_transposed_and_non_transposed_agree():
+    """The driver's two bias paths (miniafqmc_base.cpp:256-276) give the same walk:
+    exactly (to rounding) without the rotation cutoff, within its effect with it."""
+    s, p = m.preset_system("N16_M64")
+    nw = 4
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, 6)
+    a = orc.run(s, nw, 2, 3, 2, x, spT=orc.halfrotate(s, cutoff=-1.0))
+    b = orc.run(s, nw, 2, 3, 2, x, transposed=False)
+    c = orc.run(s, nw, 2, 3, 2, x, spT=orc.halfrotate(s, cutoff=1e-6))
+    assert np.abs(a[1] - b[1]).max() < 1e-12 * np.abs(b[1]).max()
+    assert np.abs(a[0] - b[0]).max() < 1e-12 * np.abs(b[0]).max()
+    dc = np.abs(c[1] - b[1]).max() / np.abs(b[1]).max()
+    assert 0 < dc < 1e-5
