@@ -39,7 +39,6 @@ MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128"
 # counters, scaled by a 1 GiB copy), per (call site, workload, mode);
 # profiles/walker_step_N128_M512_hbm_traffic.md has the commands and figures.
 MEASURED_TRAFFIC = {
-    ("prop_taylor", "N128_M512", "mixed_bf16"): 6.51e9,
 }
 
 
