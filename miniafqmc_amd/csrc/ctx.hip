@@ -116,6 +116,8 @@ struct afqmc_ctx {
   void* Vx = nullptr;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
+  void* vws = nullptr;    // split-k partials of the bias contraction
+  size_t vws_bytes = 0;
   // orthogonalisation: Cholesky QR scratch (allocated at the first call)
   bool ortho_householder = false;  // always use the blocked Householder kernel
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
@@ -531,29 +533,72 @@ int do_overlaps(afqmc_ctx* c)
   return AFQMC_OK;
 }
 
-// vbias(w, n) = sum_{s,j} G(w, s, j) Lt(s, n, j) in dense modes.
+// vbias(w, n) = sum_{s,j} G(w, s, j) Lt(s, n, j) in dense modes.  With few
+// output tiles (walkers x vectors) the long j sum is split into S chunks:
+// the 2S partial products (spin, chunk) run as one batched GEMM into a
+// workspace and are then summed in FP64.
 template <int CP, class GT, class LT_>
 int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, const char* name)
 {
-  const long NM = (long)c->N * c->M;
-  for (int sp = 0; sp < 2; sp++) {
-    ZGemmArgs a = gargs();
-    a.A = zadv<GT>(Gp, sp * NM);
-    a.sam = 2 * NM;
-    a.sak = 1;
-    a.B = zadv<LT_>(Lt, sp * (long)c->K * NM);
-    a.sbn = NM;
-    a.sbk = 1;
-    a.C = out;
-    a.scm = c->K;
-    a.scn = 1;
-    a.m = c->nw;
-    a.n = c->K;
-    a.k = (int)NM;
-    a.br = sp == 0 ? 0.0 : 1.0;
-    RET((zgemm<CP, GT, LT_, double, true, true>(c, name, a, 1)));
+  const long NM = (long)c->N * c->M, nw = c->nw, K = c->K;
+  const long tiles = ((nw + 127) / 128) * ((K + 63) / 64);
+  int S = 1;
+  while (S < 16 && 2 * S * tiles < 2048 && NM % (2 * S) == 0 && (NM / (2 * S)) % 64 == 0) S *= 2;
+  if (S == 1) {
+    for (int sp = 0; sp < 2; sp++) {
+      ZGemmArgs a = gargs();
+      a.A = zadv<GT>(Gp, sp * NM);
+      a.sam = 2 * NM;
+      a.sak = 1;
+      a.B = zadv<LT_>(Lt, sp * K * NM);
+      a.sbn = NM;
+      a.sbk = 1;
+      a.C = out;
+      a.scm = K;
+      a.scn = 1;
+      a.m = (int)nw;
+      a.n = (int)K;
+      a.k = (int)NM;
+      a.br = sp == 0 ? 0.0 : 1.0;
+      RET((zgemm<CP, GT, LT_, double, true, true>(c, name, a, 1)));
+    }
+    return AFQMC_OK;
   }
-  return AFQMC_OK;
+  // partials in FP32 for the FP32/BF16 contractions, FP64 otherwise
+  using PT = typename std::conditional<CP == CP_F64, double, float>::type;
+  using PZ = typename ztype<PT>::type;
+  const size_t need = sizeof(PZ) * 2 * S * nw * K;
+  if (c->vws_bytes < need) {
+    dfree(c, c->vws);
+    c->vws_bytes = need;
+    RET(dalloc_bytes(c, &c->vws, need));
+  }
+  const long chunk = NM / S;
+  ZGemmArgs a = gargs();
+  a.A = Gp;  // batch b = (spin, chunk): A offset spin*NM + chunk*NM/S
+  a.sam = 2 * NM;
+  a.sak = 1;
+  a.sa1 = NM;
+  a.sa2 = chunk;
+  a.B = Lt;
+  a.sbn = NM;
+  a.sbk = 1;
+  a.sb1 = K * NM;
+  a.sb2 = chunk;
+  a.C = c->vws;
+  a.scm = K;
+  a.scn = 1;
+  a.sc1 = S * nw * K;
+  a.sc2 = nw * K;
+  a.bdiv = S;
+  a.m = (int)nw;
+  a.n = (int)K;
+  a.k = (int)chunk;
+  RET((zgemm<CP, GT, LT_, PT, true, true>(c, name, a, 2 * S)));
+  return launch(c, "vbias_reduce", [&]() {
+    hipLaunchKernelGGL((zsum_slices_kernel<PZ>), dim3(nblk(nw * K, 256)), dim3(256), 0, c->stream, (const PZ*)c->vws, out,
+                       nw * K, 2 * S);
+  });
 }
 
 // afqmc_sys::calculate_mixed_density_matrix(W, W_data, G, false)

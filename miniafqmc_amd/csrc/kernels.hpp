@@ -232,6 +232,19 @@ __global__ void zconvert_kernel(const IT* __restrict__ in, OT* __restrict__ out,
   }
 }
 
+// out[i] = sum_s in[s*n + i] (FP64 accumulation, fixed order)
+template <class PZ>
+__global__ void zsum_slices_kernel(const PZ* __restrict__ in, zd* __restrict__ out, long n, int ns)
+{
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long st = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    zd acc{0.0, 0.0};
+    for (int q = 0; q < ns; q++) acc = zadd(acc, to_zd(in[(long)q * n + i]));
+    out[i] = acc;
+  }
+}
+
 __device__ inline zd ztwice(zd v) { return zd{2.0 * v.re, 2.0 * v.im}; }
 __device__ inline zf ztwice(zf v) { return zf{2.0f * v.re, 2.0f * v.im}; }
 __device__ inline uint32_t bf2_twice(uint32_t w)
