@@ -75,6 +75,7 @@ def main():
                     help="mixed modes: 0 energy in the contraction precision, 1 FP64 G/one-body/Coulomb, "
                          "2 and FP32 exchange, 3 and FP64 exchange")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
+    ap.add_argument("--graphs", type=int, default=1, help="replay substeps from captured HIP graphs (0: eager)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
                          "hardware byte counters)")
@@ -146,22 +147,38 @@ def main():
             g.tune(s.dt)
             print(f"[bench] {mode}: tile autotune {time.perf_counter() - tt:.1f}s (untimed)", flush=True)
         g.init_walkers()
+        if args.graphs:
+            g.use_graphs(True)
         g.calculate_mixed_density_matrix()
         if not args.no_energy:
             g.calculate_energy()
         for t in range(args.warmup):
             g.substep(t, t > 0 and t % northo == 0, s.dt)
         g.sync()
-        if profile:
+        # HIP-event kernel timing runs eagerly: with graphs the timed block
+        # replays graphs without events, and a profiled eager block of the
+        # same substeps follows it (untimed for the value)
+        eager_prof = profile and not args.graphs
+        if eager_prof:
             g.profile(True)
         ta = time.perf_counter()
         eav = g.run(1, args.steps, northo, s.dt, first=args.warmup, energy=not args.no_energy)
         g.sync()
         tb = time.perf_counter()
+        out = {"Wd": g.W_data}
+        if profile and args.graphs:
+            st = (g.W, out["Wd"], g.eshift())
+            g.profile(True)
+            g.run(1, args.steps, northo, s.dt, first=args.warmup, energy=not args.no_energy)
+            g.sync()
+            # restore the timed run's end state for the probes below
+            g.set(nat.BUF_W, st[0])
+            g.set(nat.BUF_WDATA, st[1])
+            g.set_eshift(st[2])
         stats = g.kernel_stats() if profile else {}
         if profile:
             g.profile(False)
-        out = {"Wd": g.W_data, "stats": stats}
+        out["stats"] = stats
         if not args.no_energy and args.fp64:
             if state is None:
                 state = (g.W, out["Wd"], g.eshift())
@@ -262,6 +279,7 @@ def main():
         "data": "synthetic (seeded Hamiltonian and auxiliary fields)",
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
+                   "graphs": bool(args.graphs),
                    "energy": "none" if args.no_energy else f"once per {args.steps} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

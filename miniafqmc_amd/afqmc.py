@@ -199,6 +199,10 @@ class AfqmcSys:
                                       _dp(eav) if energy else None))
         return eav[:nsteps]
 
+    def use_graphs(self, enable=True):
+        """Replay each substep from a captured HIP graph (with and without orthogonalisation)."""
+        self._check(nat.lib.afqmc_use_graphs(self.ctx, 1 if enable else 0))
+
     def tune(self, dt):
         """Time every GEMM tile candidate once per call site; re-initialise walkers afterwards."""
         self._check(nat.lib.afqmc_tune(self.ctx, dt))

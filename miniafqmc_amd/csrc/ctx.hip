@@ -142,6 +142,8 @@ struct afqmc_ctx {
   long g_substep = -1;
   double g_dt = 0.0;
   double* d_dt = nullptr;
+  long* d_sub = nullptr;  // substep index read by the captured fields kernel
+  hipGraph_t graph[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -669,28 +671,29 @@ int do_vbias(afqmc_ctx* c)
   return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
 }
 
-int do_fields(afqmc_ctx* c, const double* xdev)
+// xslice != null: x is the staged block and *xslice the substep index
+int do_fields(afqmc_ctx* c, const double* xdev, const long* xslice = nullptr)
 {
   const int nw = c->nw, K = c->K;
   dim3 grid((nw + 63) / 64);
   if (c->mode == AFQMC_FP64_CSR)
     return launch(c, "fields", [&]() {
       hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, (long)nw, 1L, xdev,
-                         (zd*)c->X, (long)nw, 1L, c->hybridW, c->flags + 1);
+                         (zd*)c->X, (long)nw, 1L, c->hybridW, c->flags + 1, xslice);
     });
   if (!c->mixed)
     return launch(c, "fields", [&]() {
       hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                         (zd*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+                         (zd*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
     });
   if (c->bsplit)
     return launch(c, "fields", [&]() {
       hipLaunchKernelGGL((fields_kernel<zd, zb>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                         (zb*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+                         (zb*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
     });
   return launch(c, "fields", [&]() {
     hipLaunchKernelGGL((fields_kernel<zd, zf>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                       (zf*)c->X, 1L, (long)K, c->hybridW, c->flags + 1);
+                       (zf*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
   });
 }
 
@@ -911,7 +914,9 @@ int do_ortho(afqmc_ctx* c)
   }
   zd* X1 = c->oX;
   zd* X2 = c->oX + (long)nb * M * N;
-  RET(launch(c, "ortho_flags", [&]() { (void)hipMemsetAsync(c->ofail, 0, sizeof(int) * nb, c->stream); }));
+  RET(launch(c, "ortho_flags", [&]() {
+    hipLaunchKernelGGL(fill_int_kernel, dim3((nb + 255) / 256), dim3(256), 0, c->stream, c->ofail, nb, 0);
+  }));
   RET(cholqr_pass(c, c->W, X1));
   RET(cholqr_pass(c, X1, X2));
   RET(launch(c, "ortho_sign", [&]() {
@@ -1175,7 +1180,7 @@ int check_flags(afqmc_ctx* c)
   return AFQMC_OK;
 }
 
-int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt)
+int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt, const long* xslice = nullptr)
 {
   if (c->transposed) {
     RET(do_mdm(c));
@@ -1184,7 +1189,7 @@ int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt)
     RET(do_mdm_full(c));
     RET(do_vbias_full(c));
   }
-  RET(do_fields(c, xdev));
+  RET(do_fields(c, xdev, xslice));
   RET(do_vhs(c));
   RET(do_propagate(c));
   RET(do_save_overlaps(c));
@@ -1233,6 +1238,57 @@ void htranspose(const zd* in, zd* out, long R, long Cc)
 {
   for (long r = 0; r < R; r++)
     for (long cc = 0; cc < Cc; cc++) out[cc * R + r] = in[r * Cc + cc];
+}
+
+// Drop the captured graphs (the staged block, dt or a buffer changed).
+void drop_graphs(afqmc_ctx* c)
+{
+  for (int i = 0; i < 2; i++) {
+    if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+    if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
+    c->gexec[i] = nullptr;
+    c->graph[i] = nullptr;
+  }
+}
+
+// Capture one substep (without / with orthogonalisation) on the context's
+// stream.  Buffers the substep would allocate on first use are allocated
+// here, outside the capture.  The graph starts from the inverses of the
+// current walkers' overlap matrices (the state every substep ends in), so
+// its density matrix reuses them.
+int capture_substep(afqmc_ctx* c, int ortho, double dt)
+{
+  const long MN = (long)c->M * c->N, nb = 2L * c->nw;
+  if (!c->d_sub) RET(dalloc(c, &c->d_sub, 1));
+  if (ortho && c->N <= 128 && !c->ortho_householder && !c->oU) {
+    RET(dalloc(c, &c->oU, nb * c->N * c->N));
+    RET(dalloc(c, &c->oX, 2L * nb * MN));
+    RET(dalloc(c, &c->ofail, nb));
+  }
+  if (!c->transposed && !c->Gfull) RET(dalloc(c, &c->Gfull, 2L * c->M * c->M * c->nw));
+  if (c->mode != AFQMC_FP64_CSR) {
+    // the bias contraction's split-k workspace at its largest
+    const size_t need = sizeof(zd) * 32 * (size_t)c->nw * c->K;
+    if (c->vws_bytes < need) {
+      dfree(c, c->vws);
+      c->vws_bytes = need;
+      RET(dalloc_bytes(c, &c->vws, need));
+    }
+  }
+  if (!c->t1inv_valid) RET(do_overlaps(c));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r != AFQMC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph[ortho] = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec[ortho], g, nullptr, nullptr, 0));
+  return AFQMC_OK;
 }
 
 } // namespace
@@ -1313,6 +1369,8 @@ int afqmc_ctx_destroy(afqmc_ctx* c)
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (int i = 0; i < 2; i++)
     if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
+  for (int i = 0; i < 2; i++)
+    if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
   for (auto& p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -1354,6 +1412,7 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {
+    drop_graphs(c);
     c->ortho_householder = value != 0.0;
     return AFQMC_OK;
   }
@@ -1790,6 +1849,7 @@ int afqmc_stage_fields(afqmc_ctx* c, const double* x_all, long nsub)
     c->xstage = nullptr;
     c->nstage = 0;
   }
+  drop_graphs(c);  // the captured substeps read the old block
   RET(dalloc(c, &c->xstage, KW * nsub));
   HIPCHK(c, hipMemcpyAsync(c->xstage, x_all, sizeof(double) * KW * nsub, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1802,7 +1862,22 @@ int afqmc_substep(afqmc_ctx* c, long substep, int do_ortho_, double dt)
   NEED_SYSTEM(c);
   if (!c->xstage || substep < 0 || substep >= c->nstage) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
   const long KW = (long)c->K * c->nw;
-  return substep_body(c, c->xstage + substep * KW, do_ortho_, dt);
+  const int o = do_ortho_ ? 1 : 0;
+  if (!c->use_graphs || c->prof || c->tuning) return substep_body(c, c->xstage + substep * KW, do_ortho_, dt);
+  if (c->g_dt != dt) {
+    drop_graphs(c);
+    c->g_dt = dt;
+  }
+  if (!c->gexec[o]) RET(capture_substep(c, o, dt));
+  // the graph reuses the inverses of the current walkers (see capture_substep)
+  if (!c->t1inv_valid) RET(do_overlaps(c));
+  RET(launch(c, "set_substep", [&]() {
+    hipLaunchKernelGGL(set_long_kernel, dim3(1), dim3(64), 0, c->stream, c->d_sub, substep);
+  }));
+  HIPCHK(c, hipGraphLaunch(c->gexec[o], c->stream));
+  c->t1inv_valid = true;  // the substep ends with the overlaps of the new walkers
+  c->g_fresh = false;
+  return AFQMC_OK;
 }
 
 int afqmc_get_eshift(afqmc_ctx* c, double* e)
@@ -1845,6 +1920,7 @@ int afqmc_tune(afqmc_ctx* c, double dt)
   NEED_SYSTEM(c);
   if (!c->xstage || c->nstage < 1) return fail(c, AFQMC_ERR_STATE, "stage fields before tuning");
   // tune on walkers at the trial (a representative, well-conditioned state)
+  drop_graphs(c);
   int r = afqmc_init_walkers(c);
   if (r != AFQMC_OK) return r;
   c->tuning = true;
@@ -1866,8 +1942,8 @@ int afqmc_tune(afqmc_ctx* c, double dt)
 int afqmc_use_graphs(afqmc_ctx* c, int enable)
 {
   if (!c) return AFQMC_ERR_ARG;
-  if (enable) return fail(c, AFQMC_ERR_UNSUPPORTED, "graph capture not built yet");
-  c->use_graphs = false;
+  drop_graphs(c);
+  c->use_graphs = enable != 0;
   return AFQMC_OK;
 }
 

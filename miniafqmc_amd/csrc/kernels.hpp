@@ -232,6 +232,16 @@ __global__ void zconvert_kernel(const IT* __restrict__ in, OT* __restrict__ out,
   }
 }
 
+__global__ void fill_int_kernel(int* __restrict__ p, int n, int v)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void set_long_kernel(long* __restrict__ p, long v)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // out[i] = sum_s in[s*n + i] (FP64 accumulation, fixed order)
 template <class PZ>
 __global__ void zsum_slices_kernel(const PZ* __restrict__ in, zd* __restrict__ out, long n, int ns)
@@ -294,8 +304,12 @@ __global__ void zaxpy1_kernel(const zd* __restrict__ x, zd* __restrict__ y, long
 template <class VT, class XT>
 __global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __restrict__ vb, long vsn, long vsw,
                                                      const double* __restrict__ x, XT* __restrict__ X, long xsn,
-                                                     long xsw, zd* __restrict__ hybridW, int* __restrict__ nonfinite)
+                                                     long xsw, zd* __restrict__ hybridW, int* __restrict__ nonfinite,
+                                                     const long* __restrict__ xslice)
 {
+  // xslice != null: the draws of substep *xslice of the staged block (the
+  // index lives in device memory so a captured graph can be replayed)
+  if (xslice) x += *xslice * (long)K * nw;
   __shared__ zd part[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;

@@ -324,3 +324,36 @@ def test_run_non_transposed_matches_oracle(c1):
     close(np.concatenate([[e0], eav]), eo, 1e-9, "Eav trajectory")
     close(W, Wo, 1e-9, "final walkers")
     close(Wd[:, :4], Wdo[:, :4], 1e-9, "final W_data")
+
+
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense", "mixed_bf16"])
+def test_graph_replay_matches_eager(c1, mode):
+    """Substeps replayed from captured graphs equal the eager substeps, replay after replay,
+    with eager work (energy, copies) between the replays and both graph kinds (with ortho)."""
+    s, p = c1
+    nw, nsub = p["nwalk"], 7
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, nsub)
+    ctxs = []
+    for graphs in (False, True):
+        g = m.AfqmcSys(s, nw, mode)
+        g.init_walkers()
+        g.stage_fields(x)
+        if graphs:
+            g.use_graphs(True)
+        ctxs.append(g)
+    say("graph replay", mode)
+    for t in range(nsub):
+        ortho = t > 0 and t % 3 == 0
+        for g in ctxs:
+            g.substep(t, ortho, s.dt)
+        Wa, Wb = ctxs[0].W, ctxs[1].W
+        close(Wb, Wa, 1e-13, f"walkers after substep {t}")
+        close(ctxs[1].W_data[:, :5], ctxs[0].W_data[:, :5], 1e-13, f"W_data after substep {t}")
+        if t == 4:  # eager work between replays
+            for g in ctxs:
+                g.calculate_mixed_density_matrix()
+                g.calculate_energy()
+            close(ctxs[1].W_data[:, 0], ctxs[0].W_data[:, 0], 1e-13, "E_L between replays")
+    assert abs(ctxs[1].eshift() - ctxs[0].eshift()) <= 1e-13 * abs(ctxs[0].eshift())
+    for g in ctxs:
+        g.close()
