@@ -71,9 +71,14 @@ def main():
     ap.add_argument("--cpu", type=int, default=1, help="time the oracle on a walker sample (0 to skip)")
     ap.add_argument("--cpu-walkers", type=int, default=6)
     ap.add_argument("--no-energy", action="store_true")
-    ap.add_argument("--energy-level", type=int, default=3,
+    ap.add_argument("--energy-level", type=int, default=2,
                     help="mixed modes: 0 energy in the contraction precision, 1 FP64 G/one-body/Coulomb, "
-                         "2 and FP32 exchange, 3 and FP64 exchange")
+                         "2 and FP32 exchange (FP64 sums over column chunks), 3 and FP64 exchange")
+    ap.add_argument("--exchange-chunk", type=int, default=2048,
+                    help="mixed levels 0-2: exchange columns per FP32-accumulated chunk (0: one pass)")
+    ap.add_argument("--substeps-per-energy", type=int, default=0,
+                    help="substeps per driver step (the local energy follows each step); 0: one step "
+                         "of all timed substeps")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     ap.add_argument("--graphs", type=int, default=1, help="replay substeps from captured HIP graphs (0: eager)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -110,6 +115,9 @@ def main():
     M, N, K = s.nmo, s.naea, s.nchol
     t_gen = time.time() - t0
     nsub = args.warmup + args.steps
+    spe = args.substeps_per_energy or args.steps
+    if args.steps % spe:
+        sys.exit("bench.py: --steps must be a multiple of --substeps-per-energy")
     x = m.fields_block(p["seed"], K, nw, 0, nsub)
     print(f"[bench] system M={M} N={N} K={K} nnz={s.spvn.nnz} walkers={nw} generated in {t_gen:.1f}s", flush=True)
 
@@ -141,6 +149,7 @@ def main():
         """Time `args.steps` substeps in `mode`; then run the accuracy probes
         from `state`, or from this run's own final walkers (returned)."""
         g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level)
+        g.set_exchange_chunk(args.exchange_chunk)
         g.stage_fields(x)
         if args.tune:
             tt = time.perf_counter()
@@ -162,14 +171,14 @@ def main():
         if eager_prof:
             g.profile(True)
         ta = time.perf_counter()
-        eav = g.run(1, args.steps, northo, s.dt, first=args.warmup, energy=not args.no_energy)
+        eav = g.run(args.steps // spe, spe, northo, s.dt, first=args.warmup, energy=not args.no_energy)
         g.sync()
         tb = time.perf_counter()
         out = {"Wd": g.W_data}
         if profile and args.graphs:
             st = (g.W, out["Wd"], g.eshift())
             g.profile(True)
-            g.run(1, args.steps, northo, s.dt, first=args.warmup, energy=not args.no_energy)
+            g.run(args.steps // spe, spe, northo, s.dt, first=args.warmup, energy=not args.no_energy)
             g.sync()
             # restore the timed run's end state for the probes below
             g.set(nat.BUF_W, st[0])
@@ -279,8 +288,9 @@ def main():
         "data": "synthetic (seeded Hamiltonian and auxiliary fields)",
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
+                   "exchange_chunk": args.exchange_chunk if args.energy_level < 3 else None,
                    "graphs": bool(args.graphs),
-                   "energy": "none" if args.no_energy else f"once per {args.steps} substeps",
+                   "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

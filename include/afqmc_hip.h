@@ -107,7 +107,15 @@ typedef enum {
    * (default): compact G and the half-rotated SpvnT for the bias.  0: full G
    * and T(Spvn) (AFQMC_FP64_CSR only; set before afqmc_upload_system, which
    * then also builds the transposed CSR of Spvn). */
-  AFQMC_OPT_TRANSPOSED = 3
+  AFQMC_OPT_TRANSPOSED = 3,
+  /* dense exchange in FP32 or split-BF16 (mixed levels 0-2) only: the
+   * exchange GEMM sums the columns of the exchange block in chunks of this
+   * many (a positive multiple of 256; default 2048), each chunk accumulated
+   * in FP32 on the matrix cores and the chunks' contributions to the energy
+   * added in FP64, which bounds the FP32 accumulation error.  0: one pass
+   * over all columns.  May be changed at any time.  No counterpart in the
+   * reference (its energy is FP64 throughout). */
+  AFQMC_OPT_EXCHANGE_CHUNK = 4
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

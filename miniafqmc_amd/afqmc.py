@@ -149,6 +149,11 @@ class AfqmcSys:
         """Always use the blocked Householder kernel for the orthogonalisation."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ORTHO_HOUSEHOLDER, 1.0 if on else 0.0))
 
+    def set_exchange_chunk(self, cols):
+        """Dense exchange: accumulate the exchange GEMM over chunks of `cols`
+        columns, adding the chunks' energies in FP64 (0: one pass)."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_EXCHANGE_CHUNK, float(cols)))
+
     def orthogonalize(self):
         """afqmc_sys::orthogonalize (afqmc_sys.hpp:175-207)."""
         self._check(nat.lib.afqmc_orthogonalize(self.ctx))

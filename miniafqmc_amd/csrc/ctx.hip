@@ -120,6 +120,7 @@ struct afqmc_ctx {
   size_t vws_bytes = 0;
   // orthogonalisation: Cholesky QR scratch (allocated at the first call)
   bool ortho_householder = false;  // always use the blocked Householder kernel
+  long exch_chunk = 2048;          // dense exchange: columns per accumulation chunk (0: one pass)
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
   zd* oX = nullptr;   // [2][2nw][M][N] the two Cholesky QR passes
   int* ofail = nullptr;  // [2nw] matrices left to the Householder kernel
@@ -957,29 +958,37 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv, const void* Lx
     // exchange from the dense same-spin block of Vakbl held in HBM as its
     // lower triangle with the strictly lower part doubled:
     // Z(w, p) = sum_{q <= p} G_s(w, q) Vx'_s(p, q);  e_x(w, s) = sum_p G_s(w, p) Z(w, p)
+    // With a chunk size set, the columns q are split into chunks [q0, q0 + kc):
+    // each chunk's GEMM accumulates over kc columns only (only p >= q0 meet
+    // the triangle) and its dot with G is added in FP64, which bounds the
+    // accumulation error of reduced-precision contractions.
     using ZT = typename ztype<YT>::type;
+    const long ch = (CP != CP_F64 && c->exch_chunk > 0 && c->exch_chunk < NM) ? c->exch_chunk : NM;
     for (int sp = 0; sp < 2; sp++) {
-      ZGemmArgs a = gargs();
-      a.A = zadv<GT>(Gp, sp * NM);
-      a.sam = 2 * NM;
-      a.sak = 1;
-      a.B = (const typename ztype<VXT>::type*)c->Vx + sp * NM * NM;
-      a.sbn = NM;
-      a.sbk = 1;
-      a.C = c->Ybuf;
-      a.scm = NM;
-      a.scn = 1;
-      a.m = nw;
-      a.n = (int)NM;
-      a.k = (int)NM;
-      a.ar = inv_dt;
-      a.ktri = 1;
-      RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
-      RET(launch(c, "energy_exch_dot", [&]() {
-        hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream, (int)NM,
-                           (const typename ztype<GT>::type*)Gp + sp * NM, 1L, 2 * NM, (const ZT*)c->Ybuf, 1L, NM,
-                           c->e_exch_part + sp, 2L, 1.0, 0);
-      }));
+      for (long q0 = 0; q0 < NM; q0 += ch) {
+        const long kc = std::min(ch, NM - q0);
+        ZGemmArgs a = gargs();
+        a.A = zadv<GT>(Gp, sp * NM + q0);
+        a.sam = 2 * NM;
+        a.sak = 1;
+        a.B = (const typename ztype<VXT>::type*)c->Vx + sp * NM * NM + q0 * NM + q0;
+        a.sbn = NM;
+        a.sbk = 1;
+        a.C = (ZT*)c->Ybuf + q0;
+        a.scm = NM;
+        a.scn = 1;
+        a.m = nw;
+        a.n = (int)(NM - q0);
+        a.k = (int)kc;
+        a.ar = inv_dt;
+        a.ktri = 1;
+        RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+        RET(launch(c, "energy_exch_dot", [&]() {
+          hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream,
+                             (int)(NM - q0), (const typename ztype<GT>::type*)Gp + sp * NM + q0, 1L, 2 * NM,
+                             (const ZT*)c->Ybuf + q0, 1L, NM, c->e_exch_part + sp, 2L, 1.0, q0 > 0 ? 1 : 0);
+        }));
+      }
     }
   } else {
   // exchange: Y_n,s = Lt_s(n) G_s^T (N x N), sum_ab Y_ab Y_ba, for all
@@ -1409,6 +1418,12 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     if (value == 0.0 && c->mode != AFQMC_FP64_CSR)
       return fail(c, AFQMC_ERR_UNSUPPORTED, "the non-transposed bias is built for AFQMC_FP64_CSR only");
     c->transposed = value != 0.0;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_EXCHANGE_CHUNK) {
+    if (value < 0 || value != (long)value || ((long)value % 256) != 0)
+      return fail(c, AFQMC_ERR_ARG, "exchange chunk must be 0 or a positive multiple of 256");
+    c->exch_chunk = (long)value;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {

@@ -6,9 +6,12 @@
 // reference's input file is not shipped and dt is hard-coded there.  The
 // auxiliary fields come from the counter-based generator (seeded), staged
 // on the device one block at a time.
+#include "afqmc_h5.h"
 #include "afqmc_hip.h"
 
+#include <dlfcn.h>
 #include <getopt.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cmath>
@@ -29,13 +32,13 @@ void print_help()
   std::printf("-s                Number of substeps (default: 10)\n");
   std::printf("-w                Number of walkers (default: 16)\n");
   std::printf("-o                Number of substeps between orthogonalization (default: 10)\n");
-  std::printf("-f                Input file name (HDF5 input is not built; use --system)\n");
+  std::printf("-f                Input file name (afqmc.h5 format; read through libafqmc_h5.so)\n");
   std::printf("-t                yes|no: half-rotated transposed Cholesky matrix for the bias potential\n");
   std::printf("                  ('no': full G and T(Spvn), fp64_csr mode only)\n");
   std::printf("-v                Verbose output (per-kernel timers)\n");
   std::printf("--dt D            time step (default 0.005)\n");
   std::printf("--mode M          fp64_csr | fp64_dense | mixed_fp32 | mixed_bf16 (default fp64_csr)\n");
-  std::printf("--system S        N16_M64 | N64_M256 | N128_M512 | MxNxK (synthetic; default N16_M64)\n");
+  std::printf("--system S        N16_M64 | N64_M256 | N128_M512 | MxNxK (synthetic, when no -f; default N16_M64)\n");
   std::printf("--density R       kept fraction of Cholesky entries for MxNxK systems (default 0.3)\n");
   std::printf("--seed S          seed of the synthetic system and fields (default 20262202)\n");
   std::printf("--tune            autotune GEMM tiles before the run\n");
@@ -79,6 +82,43 @@ int mode_of(const std::string& s)
     }                                                                                        \
   } while (0)
 
+// The HDF5 reader lives in libafqmc_h5.so (host code over the HDF5 C
+// library), loaded only when an input file is given so that the driver runs
+// where HDF5 is absent.
+struct H5Api {
+  void* so = nullptr;
+  decltype(&afqmc_h5_read) read = nullptr;
+  decltype(&afqmc_h5_get) get = nullptr;
+  decltype(&afqmc_h5_destroy) destroy = nullptr;
+  decltype(&afqmc_h5_error) error = nullptr;
+};
+
+bool load_h5(H5Api& api)
+{
+  char exe[4096];
+  const ssize_t n = readlink("/proc/self/exe", exe, sizeof(exe) - 1);
+  std::string dir = ".";
+  if (n > 0) {
+    exe[n] = 0;
+    dir = std::string(exe);
+    dir = dir.substr(0, dir.find_last_of('/'));
+  }
+  for (const std::string& cand : {dir + "/../libafqmc_h5.so", std::string("libafqmc_h5.so")}) {
+    api.so = dlopen(cand.c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (api.so) break;
+  }
+  if (!api.so) {
+    std::fprintf(stderr, "Error: cannot load libafqmc_h5.so (%s); it is built where the HDF5 C library is "
+                         "installed\n", dlerror());
+    return false;
+  }
+  api.read = (decltype(api.read))dlsym(api.so, "afqmc_h5_read");
+  api.get = (decltype(api.get))dlsym(api.so, "afqmc_h5_get");
+  api.destroy = (decltype(api.destroy))dlsym(api.so, "afqmc_h5_destroy");
+  api.error = (decltype(api.error))dlsym(api.so, "afqmc_h5_error");
+  return api.read && api.get && api.destroy && api.error;
+}
+
 } // namespace
 
 int main(int argc, char** argv)
@@ -115,54 +155,65 @@ int main(int argc, char** argv)
     }
   }
   afqmc_ctx* ctx = nullptr;
-  if (!init_file.empty()) {
-    std::fprintf(stderr, "Error: HDF5 input (%s) is not built in this version; use --system.\n", init_file.c_str());
-    return 1;
-  }
   const int mode = mode_of(mode_s);
   if (mode < 0 || nsteps < 0 || nsubsteps <= 0 || nwalk <= 0 || northo <= 0 || dt <= 0) {
     print_help();
     return 1;
   }
 
-  afqmc_synth_params sp{};
-  bool found = false;
-  for (const auto& p : kPresets)
-    if (system_s == p.name) {
-      sp.nmo = p.M;
-      sp.naea = p.N;
-      sp.nchol = p.K;
-      sp.density = p.density;
-      sp.sigma_L = p.sigma > 0 ? p.sigma : sigma_for(p.M, p.K, p.density);
-      found = true;
-    }
-  if (!found) {
-    int M = 0, N = 0, K = 0;
-    if (std::sscanf(system_s.c_str(), "%dx%dx%d", &M, &N, &K) != 3) {
-      std::fprintf(stderr, "unknown system %s\n", system_s.c_str());
+  afqmc_synth_view v{};
+  afqmc_synth* syn = nullptr;
+  afqmc_h5_system* h5s = nullptr;
+  H5Api h5;
+  if (!init_file.empty()) {
+    // afqmc::Initialize (src/Matrix/initialize_serial.hpp:35-228)
+    std::printf("***********************************************************\n");
+    std::printf("                 Initializing from HDF5                    \n");
+    std::printf("***********************************************************\n");
+    if (!load_h5(h5)) return 1;
+    if (h5.read(init_file.c_str(), dt, &h5s) != AFQMC_OK) {
+      std::fprintf(stderr, "Error reading %s: %s\n", init_file.c_str(), h5.error());
       return 1;
     }
-    sp.nmo = M;
-    sp.naea = N;
-    sp.nchol = K;
-    sp.density = density;
-    sp.sigma_L = sigma_for(M, K, density);
-  }
-  sp.sigma_h = 1.0;
-  sp.dt = dt;
-  sp.seed = seed;
-  sp.with_vakbl = (mode == AFQMC_FP64_CSR && 2.0 * sp.naea * sp.nmo <= 4096) ? 1 : 0;
+    h5.get(h5s, &v);
+  } else {
+    afqmc_synth_params sp{};
+    bool found = false;
+    for (const auto& p : kPresets)
+      if (system_s == p.name) {
+        sp.nmo = p.M;
+        sp.naea = p.N;
+        sp.nchol = p.K;
+        sp.density = p.density;
+        sp.sigma_L = p.sigma > 0 ? p.sigma : sigma_for(p.M, p.K, p.density);
+        found = true;
+      }
+    if (!found) {
+      int M = 0, N = 0, K = 0;
+      if (std::sscanf(system_s.c_str(), "%dx%dx%d", &M, &N, &K) != 3) {
+        std::fprintf(stderr, "unknown system %s\n", system_s.c_str());
+        return 1;
+      }
+      sp.nmo = M;
+      sp.naea = N;
+      sp.nchol = K;
+      sp.density = density;
+      sp.sigma_L = sigma_for(M, K, density);
+    }
+    sp.sigma_h = 1.0;
+    sp.dt = dt;
+    sp.seed = seed;
+    sp.with_vakbl = (mode == AFQMC_FP64_CSR && 2.0 * sp.naea * sp.nmo <= 4096) ? 1 : 0;
 
-  std::printf("***********************************************************\n");
-  std::printf("                 Initializing (synthetic %s)              \n", system_s.c_str());
-  std::printf("***********************************************************\n");
-  afqmc_synth* syn = nullptr;
-  if (afqmc_synth_create(&sp, &syn) != AFQMC_OK) {
-    std::fprintf(stderr, "Error generating the synthetic system\n");
-    return 1;
+    std::printf("***********************************************************\n");
+    std::printf("                 Initializing (synthetic %s)              \n", system_s.c_str());
+    std::printf("***********************************************************\n");
+    if (afqmc_synth_create(&sp, &syn) != AFQMC_OK) {
+      std::fprintf(stderr, "Error generating the synthetic system\n");
+      return 1;
+    }
+    afqmc_synth_get(syn, &v);
   }
-  afqmc_synth_view v{};
-  afqmc_synth_get(syn, &v);
   const int NMO = v.nmo, NAEA = v.naea, nchol = v.nchol;
   std::printf("\n  NMO: %d  NAEA: %d  NAEB: %d\n", NMO, NAEA, NAEA);
   std::printf("  Execution details: \n    nsteps: %d\n    nsubsteps: %d\n    nwalk: %d\n    northo: %d\n", nsteps,
@@ -176,8 +227,9 @@ int main(int argc, char** argv)
   CHECK(afqmc_ctx_create(NMO, NAEA, nchol, nwalk, mode, &ctx));
   CHECK(afqmc_set_option(ctx, AFQMC_OPT_TRANSPOSED, transposed ? 1.0 : 0.0));
   CHECK(afqmc_upload_system(ctx, v.trial_alpha, v.trial_beta, v.propg1, v.haj, &v.spvn,
-                            v.vakbl.nnz > 0 ? &v.vakbl : nullptr, 1e-6, dt));
-  afqmc_synth_destroy(syn);
+                            v.vakbl.nnz > 0 && mode == AFQMC_FP64_CSR ? &v.vakbl : nullptr, 1e-6, dt));
+  if (syn) afqmc_synth_destroy(syn);
+  if (h5s) h5.destroy(h5s);
 
   // fields for one block of substeps at a time
   const long KW = (long)nchol * nwalk;

@@ -8,8 +8,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "afqmc_hip.h")).read()
+def declared_symbols(header="afqmc_hip.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(afqmc_\w+)\s*\(", text)))
 
@@ -25,6 +25,20 @@ def test_library_exports_header():
     assert sorted(nat.EXPORTED) == syms
     for s in syms:
         getattr(nat.lib, s)
+
+
+def test_h5_library_exports_header():
+    """libafqmc_h5.so (built where the HDF5 C library is installed) exports
+    every symbol afqmc_h5.h declares."""
+    import pytest
+    from miniafqmc_amd import hdf5
+    if not hdf5.available():
+        pytest.skip("libafqmc_h5.so not built (no HDF5 C library)")
+    syms = declared_symbols("afqmc_h5.h")
+    assert syms == ["afqmc_h5_destroy", "afqmc_h5_error", "afqmc_h5_get", "afqmc_h5_read", "afqmc_h5_write"]
+    out = subprocess.run(["nm", "-D", "--defined-only", hdf5._PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (afqmc_\w+)", out))
+    assert not [s for s in syms if s not in exported]
 
 
 def test_ctx_create_rejects_bad_shapes_without_gpu():

@@ -286,6 +286,43 @@ def test_mixed_energy_evaluation_error(c1, mode):
         g.close()
 
 
+@pytest.mark.parametrize("mode", ["mixed_fp32", "mixed_bf16"])
+def test_exchange_chunks(c1, mode):
+    """The dense exchange summed over column chunks of 256 (four chunks of the
+    1024 columns here, each chunk's energy added in FP64) against the one-pass
+    exchange and the FP64 oracle, for the reduced-precision levels 0-2; chunk
+    sizes that are not multiples of 256 are rejected."""
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=29)
+    wd = np.zeros((nw, 8), complex)
+    wd[:, 1] = 1
+    Wdo = wd.copy()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wdo)
+    orc.calculate_energy(s, Wdo, Go)
+    eo = Wdo[:, 0]
+    for level, tol in ((0, 1e-3), (1, 1e-5), (2, 1e-5)):
+        g = m.AfqmcSys(s, nw, mode, energy_precision=level)
+        g.set(nat.BUF_W, W)
+        g.set(nat.BUF_WDATA, wd)
+        g.calculate_mixed_density_matrix()
+        e = {}
+        for ch in (0, 256, 512):
+            g.set_exchange_chunk(ch)
+            g.calculate_energy()
+            e[ch] = g.W_data[:, 0]
+            dE = np.abs(e[ch] - eo)
+            say(f"{mode} level {level} chunk {ch}: max |dE_L| = {dE.max():.3e} Ha")
+            assert dE.max() < tol
+        # the chunks partition the same sum: one-pass and chunked agree to the
+        # contraction's own rounding
+        assert np.abs(e[256] - e[0]).max() < tol
+        assert np.abs(e[512] - e[256]).max() < tol
+        with pytest.raises(RuntimeError):
+            g.set_exchange_chunk(100)
+        g.close()
+
+
 def test_non_transposed_bias_phases(c1):
     """Full G (mixed_density_matrix.hpp:93-101) and vbias = T(Spvn) G (vbias.hpp:65-82) vs the oracle."""
     s, p = c1
