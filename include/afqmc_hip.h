@@ -175,6 +175,19 @@ int afqmc_save_overlaps(afqmc_ctx* ctx);
 int afqmc_calculate_overlaps(afqmc_ctx* ctx);
 /* weight / hybrid-energy / Eshift update (src/miniapps/miniafqmc_base.cpp:317-327) */
 int afqmc_weights(afqmc_ctx* ctx, double dt, double* eshift_inout);
+/* Population control by the comb method, for the place after a step's local
+ * energy where the reference's driver marks that branching would happen
+ * (src/miniapps/miniafqmc_base.cpp:346; the reference implements none).
+ * With w_i = max(0, Re W_data[i][1]) and W = sum_i w_i, the nwalk teeth
+ * t_j = (u + j) * W / nwalk, j = 0..nwalk-1, pick the walker whose
+ * cumulative-weight interval holds them, so walker i is copied floor or ceil
+ * of nwalk * w_i / W times, copies in index order; walkers with zero weight
+ * are dropped.  Walkers and their W_data rows move together and every new
+ * weight is W / nwalk (the population and the total weight are kept).  u is
+ * the caller's uniform draw in [0, 1); parent_out (may be NULL) receives the
+ * parent index of each new walker [nwalk].  AFQMC_ERR_NONFINITE when W is
+ * not positive or a weight is not finite (walkers untouched). */
+int afqmc_branch(afqmc_ctx* ctx, double u, int* parent_out);
 /* afqmc_sys::orthogonalize (src/AFQMC/afqmc_sys.hpp:175-207): Householder LQ
  * with LAPACK's conventions (zgelqf + zunglq) on each W[w][s]. */
 int afqmc_orthogonalize(afqmc_ctx* ctx);

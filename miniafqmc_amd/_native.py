@@ -34,7 +34,7 @@ EXPORTED = [
     "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_mixed_density_matrix_full", "afqmc_vbias_full",
     "afqmc_fields", "afqmc_vhs",
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
-    "afqmc_weights", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
+    "afqmc_weights", "afqmc_branch", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
     "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
     "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats", "afqmc_zgemm",
     "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
@@ -110,6 +110,7 @@ def _load():
         "afqmc_save_overlaps": (I, [P]),
         "afqmc_calculate_overlaps": (I, [P]),
         "afqmc_weights": (I, [P, C.c_double, D]),
+        "afqmc_branch": (I, [P, C.c_double, C.POINTER(C.c_int)]),
         "afqmc_orthogonalize": (I, [P]),
         "afqmc_energy": (I, [P, D]),
         "afqmc_stage_fields": (I, [P, D, L]),

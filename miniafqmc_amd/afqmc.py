@@ -145,6 +145,13 @@ class AfqmcSys:
         self._check(nat.lib.afqmc_weights(self.ctx, dt, C.byref(e)))
         return e.value
 
+    def branch(self, u):
+        """Comb population control (afqmc_branch): returns the parent index of
+        each new walker."""
+        parent = np.zeros(self.nwalk, np.int32)
+        self._check(nat.lib.afqmc_branch(self.ctx, float(u), parent.ctypes.data_as(C.POINTER(C.c_int))))
+        return parent
+
     def set_ortho_householder(self, on):
         """Always use the blocked Householder kernel for the orthogonalisation."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ORTHO_HOUSEHOLDER, 1.0 if on else 0.0))

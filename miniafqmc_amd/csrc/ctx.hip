@@ -90,6 +90,11 @@ struct afqmc_ctx {
   zd* dcache = nullptr;   // [nw][2] determinants of the last inversion
   bool t1inv_valid = false;  // T1 holds the inverses for the current walkers
   bool g_fresh = false;      // G was built from T1 and W as they are now
+  zd* Wbr = nullptr;         // population control: gathered walkers [nw][2][M][N]
+  zd* Wdbr = nullptr;        // and W_data rows [nw][8]
+  double* brcum = nullptr;   // cumulative weights [nw]
+  int* brparent = nullptr;   // parent of each new walker [nw]
+  double* brw = nullptr;     // common weight after the comb
   void* G = nullptr;      // CSR: zd [2NM][nw]; FP64_DENSE: zd [nw][2NM]; mixed: zf [nw][2NM]
   zd* vbias = nullptr;    // CSR: [K][nw]; dense: [nw][K]
   void* X = nullptr;      // CSR: zd [K][nw]; dense: [nw][K] (zd / zf)
@@ -816,6 +821,48 @@ int do_save_overlaps(afqmc_ctx* c)
   return launch(c, "save_overlaps", [&]() {
     hipLaunchKernelGGL(save_overlaps_kernel, dim3((c->nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->nw);
   });
+}
+
+// Comb population control (after a step's energy, where the driver marks
+// branching: src/miniapps/miniafqmc_base.cpp:346).  Walker i is copied
+// floor or ceil of nw * w_i / W times, in index order, and every new weight
+// is W / nw.
+int do_branch(afqmc_ctx* c, double u, int* parent_out)
+{
+  const int nw = c->nw;
+  const long per = 2L * c->M * c->N;
+  if (!(u >= 0.0 && u < 1.0)) return fail(c, AFQMC_ERR_ARG, "comb offset u must lie in [0, 1)");
+  if (!c->Wbr) {
+    RET(dalloc(c, &c->Wbr, (size_t)nw * per));
+    RET(dalloc(c, &c->Wdbr, (size_t)nw * 8));
+    RET(dalloc(c, &c->brcum, (size_t)nw));
+    RET(dalloc(c, &c->brparent, (size_t)nw));
+    RET(dalloc(c, &c->brw, 1));
+  }
+  HIPCHK(c, hipMemsetAsync(c->flags + 2, 0, sizeof(int), c->stream));
+  RET(launch(c, "branch_select", [&]() {
+    hipLaunchKernelGGL(comb_select_kernel, dim3(1), dim3(256), 0, c->stream, c->Wd, nw, u, c->brcum, c->brparent,
+                       c->brw, c->flags + 2);
+  }));
+  int bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->flags + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (bad) return fail(c, AFQMC_ERR_NONFINITE, "population control: total weight not positive or a weight not finite");
+  const long n = (long)nw * per;
+  const unsigned grid = (unsigned)std::min<long>(16384, std::max<long>(1, (n + 255) / 256));
+  RET(launch(c, "branch_gather", [&]() {
+    hipLaunchKernelGGL(comb_gather_kernel, dim3(grid), dim3(256), 0, c->stream, (const zd*)c->W, c->Wbr,
+                       (const zd*)c->Wd, c->Wdbr, (const int*)c->brparent, (const double*)c->brw, per, nw);
+  }));
+  HIPCHK(c, hipMemcpyAsync(c->W, c->Wbr, sizeof(zd) * n, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->Wd, c->Wdbr, sizeof(zd) * nw * 8, hipMemcpyDeviceToDevice, c->stream));
+  c->t1inv_valid = false;
+  c->g_fresh = false;
+  if (parent_out) {
+    HIPCHK(c, hipMemcpyAsync(parent_out, c->brparent, sizeof(int) * nw, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return AFQMC_OK;
 }
 
 int do_weights(afqmc_ctx* c, const double* dtp, double dt)
@@ -1842,6 +1889,11 @@ int afqmc_weights(afqmc_ctx* c, double dt, double* eshift_inout)
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return AFQMC_OK;
+}
+int afqmc_branch(afqmc_ctx* c, double u, int* parent_out)
+{
+  NEED_SYSTEM(c);
+  return do_branch(c, u, parent_out);
 }
 int afqmc_orthogonalize(afqmc_ctx* c)
 {

@@ -381,6 +381,67 @@ __global__ __launch_bounds__(1024) void weights_kernel(zd* __restrict__ Wd, cons
   if (threadIdx.x == 0) *eshift = red[0] / nw;
 }
 
+// Comb population control, selection (one workgroup).  Weights are
+// w_i = Re W_data[i][1] (the phaseless weights are >= 0; a negative weight
+// counts as 0); cum_i = w_0 + ... + w_i is summed in index order by one
+// thread so that every caller gets the same rounding; tooth j sits at
+// t_j = (u + j) * (cum_{nw-1} / nw) and takes the first walker with
+// cum_i > t_j, or the last walker of positive weight when rounding puts t_j
+// at or past the total.  flag[0] is set when the total is not positive and
+// finite or a weight is not finite.
+__global__ __launch_bounds__(1024) void comb_select_kernel(const zd* __restrict__ Wd, int nw, double u,
+                                                            double* __restrict__ cum, int* __restrict__ parent,
+                                                            double* __restrict__ wnew, int* __restrict__ flag)
+{
+  __shared__ int last_pos;
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    int lp = -1, bad = 0;
+    for (int i = 0; i < nw; i++) {
+      const double w = Wd[(long)i * 8 + 1].re;
+      if (!isfinite(w)) bad = 1;
+      if (w > 0.0) {
+        sum += w;
+        lp = i;
+      }
+      cum[i] = sum;
+    }
+    if (bad || !(sum > 0.0) || !isfinite(sum)) flag[0] = 1;
+    *wnew = sum / nw;
+    last_pos = lp;
+  }
+  __syncthreads();
+  const double total = cum[nw - 1];
+  const double step = total / nw;
+  for (int j = threadIdx.x; j < nw; j += blockDim.x) {
+    const double t = (u + j) * step;
+    int lo = 0, hi = nw;  // first i with cum[i] > t, nw if none
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cum[mid] > t) hi = mid;
+      else lo = mid + 1;
+    }
+    parent[j] = lo < nw ? lo : max(last_pos, 0);
+  }
+}
+
+// Walkers and W_data rows of the selected parents: Wout[j] = W[parent[j]],
+// Wdout[j] = Wd[parent[j]] with the weight replaced by the common weight.
+__global__ void comb_gather_kernel(const zd* __restrict__ W, zd* __restrict__ Wout, const zd* __restrict__ Wd,
+                                   zd* __restrict__ Wdout, const int* __restrict__ parent,
+                                   const double* __restrict__ wnew, long per, int nw)
+{
+  const long n = (long)nw * per;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long j = e / per, q = e - j * per;
+    Wout[e] = W[(long)parent[j] * per + q];
+  }
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)nw * 8; e += (long)gridDim.x * blockDim.x) {
+    const long j = e / 8, k = e % 8;
+    Wdout[e] = (k == 1) ? zd{*wnew, 0.0} : Wd[(long)parent[j] * 8 + k];
+  }
+}
+
 // W = conj(trial) per spin, W_data row = (0, 1, 0, ...)
 __global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, const zd* __restrict__ trial, int nw,
                                     long MN)

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -42,6 +43,7 @@ void print_help()
   std::printf("--density R       kept fraction of Cholesky entries for MxNxK systems (default 0.3)\n");
   std::printf("--seed S          seed of the synthetic system and fields (default 20262202)\n");
   std::printf("--tune            autotune GEMM tiles before the run\n");
+  std::printf("--branch          comb population control after each step's energy\n");
 }
 
 struct Preset {
@@ -125,7 +127,7 @@ int main(int argc, char** argv)
 {
   int nsteps = 10, nsubsteps = 10, nwalk = 16, northo = 10;
   double dt = 0.005;
-  bool verbose = false, tune = false, transposed = true;
+  bool verbose = false, tune = false, transposed = true, branch = false;
   std::string mode_s = "fp64_csr", system_s = "N16_M64", init_file;
   double density = 0.3;
   uint64_t seed = 20262202;
@@ -133,7 +135,7 @@ int main(int argc, char** argv)
   static option longopts[] = {{"dt", required_argument, nullptr, 1},     {"mode", required_argument, nullptr, 2},
                               {"system", required_argument, nullptr, 3}, {"density", required_argument, nullptr, 4},
                               {"seed", required_argument, nullptr, 5},   {"tune", no_argument, nullptr, 6},
-                              {nullptr, 0, nullptr, 0}};
+                              {"branch", no_argument, nullptr, 7},       {nullptr, 0, nullptr, 0}};
   int opt;
   while ((opt = getopt_long(argc, argv, "t:hvi:s:w:o:f:", longopts, nullptr)) != -1) {
     switch (opt) {
@@ -151,6 +153,7 @@ int main(int argc, char** argv)
     case 4: density = std::atof(optarg); break;
     case 5: seed = std::strtoull(optarg, nullptr, 10); break;
     case 6: tune = true; break;
+    case 7: branch = true; break;
     default: print_help(); return 1;
     }
   }
@@ -254,6 +257,8 @@ int main(int argc, char** argv)
   std::printf("# Step   Energy   \n");
   if (verbose) CHECK(afqmc_profile(ctx, 1));
   double total = 0.0;
+  std::vector<double> eav_steps;
+  std::mt19937_64 comb_rng(seed ^ 0x9e3779b97f4a7c15ULL);
   for (int step = 0; step < nsteps; step++) {
     const long first = (long)step * nsubsteps;
     CHECK(stage_block(first));
@@ -266,8 +271,21 @@ int main(int argc, char** argv)
     }
     CHECK(afqmc_mixed_density_matrix(ctx));
     CHECK(afqmc_energy(ctx, &Eav));
+    // branching goes here (miniafqmc_base.cpp:346): comb with a uniform offset
+    if (branch) CHECK(afqmc_branch(ctx, (double)(comb_rng() >> 11) * 0x1.0p-53, nullptr));
     total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("%d   %.10f\n", step, Eav);
+    eav_steps.push_back(Eav);
+  }
+  if (eav_steps.size() > 1) {
+    // mixed estimator over the steps: mean and standard error of the step energies
+    double mean = 0, var = 0;
+    for (double e : eav_steps) mean += e;
+    mean /= eav_steps.size();
+    for (double e : eav_steps) var += (e - mean) * (e - mean);
+    var /= (eav_steps.size() - 1);
+    std::printf("\nMixed estimator over %zu steps: %.10f +/- %.3e\n", eav_steps.size(), mean,
+                std::sqrt(var / eav_steps.size()));
   }
   std::printf("\n***********************************************************\n");
   std::printf("                   Finished Calculation                    \n");

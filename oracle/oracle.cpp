@@ -26,6 +26,12 @@
  *   calculate_energy         Vakbl form == Cholesky form (1e-12), brute force E1
  *   fields / weights         hand-made closed-form rows
  *   halfrotate_cholesky      dense brute force with cutoff
+ *   orc_branch               no reference implementation (the driver only marks
+ *                            the place, miniafqmc_base.cpp:346): a hand-worked
+ *                            comb, and the comb's invariants on random weights
+ *                            (copies of walker i = floor or ceil of nw*w_i/W,
+ *                            population nw, parents non-decreasing, zero-weight
+ *                            walkers dropped, total weight kept)
  *   orc_compress             sorted (row, col) order + rowIndex by brute force
  *   orc_run                  composition of the pinned pieces; no end-to-end
  *                            reference fixture exists (the example afqmc.h5 is
@@ -755,6 +761,50 @@ int orc_fields(int K, int nw, const double* vbias, const double* x, double* X, d
 int orc_weights(int nw, double dt, const double* hybridW, double* Wd, double* eshift_inout)
 {
   *eshift_inout = weights(nw, dt, CCZ(hybridW), CZ(Wd), *eshift_inout);
+  return 0;
+}
+
+/* Comb population control at the place the driver marks for branching
+ * (miniafqmc_base.cpp:346).  Cumulative weights max(0, Re Wd[i][1]) in index
+ * order; tooth j at (u + j) * (W / nw) takes the first walker whose cumulative
+ * weight exceeds it (the last walker of positive weight if rounding puts the
+ * tooth at or past W).  W and Wd are gathered in place; every new weight is
+ * W / nw.  Returns 1 (nothing changed) if W is not positive and finite or a
+ * weight is not finite. */
+int orc_branch(int M, int N, int nw, double u, double* W_, double* Wd_, int* parent)
+{
+  cd* W = CZ(W_);
+  cd* Wd = CZ(Wd_);
+  std::vector<double> cum(nw);
+  double sum = 0.0;
+  int last = -1;
+  bool bad = false;
+  for (int i = 0; i < nw; i++) {
+    const double w = Wd[(size_t)i * 8 + 1].real();
+    if (!std::isfinite(w)) bad = true;
+    if (w > 0.0) {
+      sum += w;
+      last = i;
+    }
+    cum[i] = sum;
+  }
+  if (bad || !(sum > 0.0) || !std::isfinite(sum)) return 1;
+  const double step = cum[nw - 1] / nw;
+  for (int j = 0; j < nw; j++) {
+    const double t = (u + j) * step;
+    int i = 0;
+    while (i < nw && !(cum[i] > t)) i++;
+    parent[j] = i < nw ? i : last;
+  }
+  const size_t per = (size_t)2 * M * N;
+  std::vector<cd> Wn((size_t)nw * per), Wdn((size_t)nw * 8);
+  for (int j = 0; j < nw; j++) {
+    std::copy(W + parent[j] * per, W + (parent[j] + 1) * per, Wn.begin() + j * per);
+    std::copy(Wd + (size_t)parent[j] * 8, Wd + (size_t)parent[j] * 8 + 8, Wdn.begin() + (size_t)j * 8);
+    Wdn[(size_t)j * 8 + 1] = cd(sum / nw, 0.0);
+  }
+  std::copy(Wn.begin(), Wn.end(), W);
+  std::copy(Wdn.begin(), Wdn.end(), Wd);
   return 0;
 }
 

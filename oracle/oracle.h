@@ -64,6 +64,8 @@ int orc_energy_cholesky(int M, int N, int K, int nw, double* Wd, const double* G
                         const double* haj, const double* Lt, double* eav_out);
 int orc_fields(int K, int nw, const double* vbias, const double* x, double* X, double* hybridW);
 int orc_weights(int nw, double dt, const double* hybridW, double* Wd, double* eshift_inout);
+/* comb population control (the driver marks the place, miniafqmc_base.cpp:346) */
+int orc_branch(int M, int N, int nw, double u, double* W, double* Wd, int* parent);
 
 /* half rotation: two-pass count + fill (rotate.hpp:65-169).  Call with
  * out arrays NULL to get nnz; then with arrays of that size. */

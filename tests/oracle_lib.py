@@ -223,6 +223,14 @@ def weights(dt, hybridW, Wd, eshift):
     return e.value
 
 
+def branch(W, Wd, u):
+    """Comb population control in place; returns (status, parent)."""
+    nw, _, M, N = W.shape
+    parent = np.zeros(nw, np.int32)
+    st = lib.orc_branch(M, N, nw, C.c_double(u), dp(W), dp(Wd), ip(parent))
+    return st, parent
+
+
 def compress(nrows, rows, cols, vals):
     rows = np.ascontiguousarray(rows, np.int32).copy()
     cols = np.ascontiguousarray(cols, np.int32).copy()

@@ -394,3 +394,71 @@ def test_graph_replay_matches_eager(c1, mode):
     assert abs(ctxs[1].eshift() - ctxs[0].eshift()) <= 1e-13 * abs(ctxs[0].eshift())
     for g in ctxs:
         g.close()
+
+
+@pytest.mark.parametrize("nw", [16, 1000])
+def test_branch_matches_oracle(c1, nw):
+    """Comb population control: parents bit-exact with the oracle's comb, the
+    walkers and W_data rows gathered exactly, the common weight W/nw; the
+    density matrix of the new walkers then matches the oracle's (the cached
+    inverses are rebuilt); degenerate weights are refused with the walkers
+    untouched."""
+    s, p = c1
+    g = m.AfqmcSys(s, nw, "fp64_csr")
+    r = np.random.default_rng(nw)
+    for u in (0.0, 0.37, 1.0 - 2.0 ** -53):
+        W = perturbed_walkers(s, nw, seed=31)
+        Wd = r.standard_normal((nw, 8)) + 1j * r.standard_normal((nw, 8))
+        Wd[:, 1] = r.exponential(1.0, nw)
+        Wd[r.random(nw) < 0.25, 1] = 0.0
+        g.set(nat.BUF_W, W)
+        g.set(nat.BUF_WDATA, Wd)
+        g.calculate_mixed_density_matrix()  # leaves cached inverses of the old walkers
+        Wd = g.W_data                        # with the overlaps it wrote
+        say(f"branch nw={nw} u={u}")
+        parent = g.branch(u)
+        Wo, Wdo = W.copy(), Wd.copy()
+        st, po = orc.branch(Wo, Wdo, u)
+        assert st == 0
+        np.testing.assert_array_equal(parent, po)
+        np.testing.assert_array_equal(g.W, Wo)
+        np.testing.assert_array_equal(g.W_data, Wdo)
+    G = g.calculate_mixed_density_matrix()
+    Go, _ = orc.calculate_mixed_density_matrix(s, Wo, Wdo.copy())
+    close(G, Go, what="G after branching")
+    Wd = g.W_data
+    Wd[:, 1] = 0.0
+    g.set(nat.BUF_WDATA, Wd)
+    W0 = g.W
+    with pytest.raises(nat.AfqmcError) as ei:
+        g.branch(0.5)
+    assert ei.value.code == 6
+    np.testing.assert_array_equal(g.W, W0)
+    g.close()
+
+
+def test_branch_between_graph_replays(c1):
+    """Substeps replayed from graphs, a comb, more replays: equal to the same
+    sequence run eagerly (the replay after the comb rebuilds the inverses)."""
+    s, p = c1
+    nw, nsub = p["nwalk"], 6
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, nsub)
+    ctxs = []
+    for graphs in (False, True):
+        g = m.AfqmcSys(s, nw, "fp64_csr")
+        g.init_walkers()
+        g.stage_fields(x)
+        if graphs:
+            g.use_graphs(True)
+        ctxs.append(g)
+    say("branch between graph replays")
+    for t in range(nsub):
+        for g in ctxs:
+            g.substep(t, t == 4, s.dt)
+        if t == 2:
+            pa, pb = (g.branch(0.61) for g in ctxs)
+            np.testing.assert_array_equal(pa, pb)
+        close(ctxs[1].W, ctxs[0].W, 1e-13, f"walkers after substep {t}")
+        close(ctxs[1].W_data[:, :5], ctxs[0].W_data[:, :5], 1e-13, f"W_data after substep {t}")
+    for g in ctxs:
+        g.close()
