@@ -462,3 +462,23 @@ def test_branch_between_graph_replays(c1):
         close(ctxs[1].W_data[:, :5], ctxs[0].W_data[:, :5], 1e-13, f"W_data after substep {t}")
     for g in ctxs:
         g.close()
+
+
+@pytest.mark.parametrize("M,N,K,nw", [(40, 7, 53, 5), (24, 1, 10, 3), (33, 12, 70, 1)])
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense", "mixed_bf16"])
+def test_run_ragged_shapes(require_gpu, M, N, K, nw, mode):
+    """Whole steps at shapes that fill no tile evenly (odd M, N, K and walker
+    counts, one electron, one walker) against the oracle: FP64 at 1e-9, the
+    split-BF16 path within 1e-3 relative on the energies."""
+    say("ragged", M, N, K, nw, mode)
+    s = m.make_system(M, N, K, density=0.5, sigma_L=0.03, sigma_h=1.0, dt=0.005, seed=M * 100 + N,
+                      with_vakbl=(mode == "fp64_csr"))
+    p = dict(nsteps=2, nsubsteps=3, northo=2, seed=M + K, nwalk=nw)
+    cutoff = 1e-6 if mode == "fp64_csr" else -1.0
+    g, o = run_both(s, p, mode, cutoff=cutoff)
+    if mode == "mixed_bf16":
+        close(g["eav"], o["eav"], 1e-3, "Eav trajectory")
+    else:
+        close(g["eav"], o["eav"], 1e-9, "Eav trajectory")
+        close(g["W"], o["W"], 1e-9, "final walkers")
+        close(g["Wd"][:, :5], o["Wd"][:, :5], 1e-9, "final W_data")
