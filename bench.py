@@ -15,6 +15,7 @@ per-walker |dE_L| between the two, a roofline entry for the dominant kernel
 oracle's CPU walker-steps/s on a walker sample.
 """
 import argparse
+import ctypes as C
 import json
 import math
 import os
@@ -42,6 +43,22 @@ MEASURED_TRAFFIC = {
     # Horner step: 4.00 GiB read (V, U, S) + 1.00 GiB written (U') per launch
     ("prop_taylor", "N128_M512", "mixed_bf16"): 5.373e9,
 }
+
+
+def kernel_bytes(name, M, N, K, nw, nnz_spvn, nnz_spvnT):
+    """Algorithmic HBM bytes per launch of the FP64 CSR products: the sparse
+    matrix once (16 B value + 4 B column per entry, 4 B row pointers), the
+    dense operand once and the output once (16 B per complex)."""
+    if name == "csrmm_vhs":
+        return nnz_spvn * 20 + (M * M + 1) * 4 + K * nw * 16 + M * M * nw * 16
+    if name == "csrmm_vbias":
+        return nnz_spvnT * 20 + (K + 1) * 4 + 2 * N * M * nw * 16 + K * nw * 16
+    return None
+
+
+def csr_flops(name, nw, nnz_spvn, nnz_spvnT):
+    c = {"csrmm_vhs": nnz_spvn * nw, "csrmm_vbias": nnz_spvnT * nw}.get(name)
+    return None if c is None else 8.0 * c
 
 
 def kernel_flops(name, M, N, K, nw):
@@ -177,6 +194,10 @@ def main():
         g.sync()
         tb = time.perf_counter()
         out = {"Wd": g.W_data}
+        if mode == "fp64_csr":
+            nnz = C.c_longlong(0)
+            nat.check(nat.lib.afqmc_get_spvnT(g.ctx, C.byref(nnz), None, None, None))
+            out["nnz_spvnT"] = nnz.value
         if profile and args.graphs:
             st = (g.W, out["Wd"], g.eshift())
             g.profile(True)
@@ -208,7 +229,20 @@ def main():
     # roofline of the dominant kernel with a known algorithmic count
     roof = None
     timed = [(k_, v) for k_, v in stats.items() if kernel_flops(k_, M, N, K, nw) is not None]
-    if timed:
+    csr = [(k_, v) for k_, v in stats.items() if k_ in ("csrmm_vhs", "csrmm_vbias")]
+    if args.mode == "fp64_csr" and csr:
+        # the reference's sparse products: report HBM bandwidth against the
+        # algorithmic bytes, beside the FP64 rate they reach
+        name, (nl, ms) = max(csr, key=lambda kv: kv[1][1])
+        avg_s = ms / nl / 1e3
+        nbytes = kernel_bytes(name, M, N, K, nw, s.spvn.nnz, res["nnz_spvnT"])
+        fl = csr_flops(name, nw, s.spvn.nnz, res["nnz_spvnT"])
+        ach = nbytes / avg_s / 1e9
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
+                "frac": round(ach / 8000.0, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
+                "avg_ms": round(ms / nl, 4), "bytes_per_launch": nbytes,
+                "fp64_tflops": round(fl / avg_s / 1e12, 2)}
+    elif timed:
         name, (nl, ms) = max(timed, key=lambda kv: kv[1][1])
         avg_s = ms / nl / 1e3
         flops = kernel_flops(name, M, N, K, nw)
@@ -276,7 +310,7 @@ def main():
               flush=True)
 
     line = {
-        "metric": "walker-steps/s (M=512, N=128+128, nchol=2048, 1024 walkers)",
+        "metric": f"walker-steps/s (M={M}, N={N}+{N}, nchol={K}, {nw} walkers)",
         "value": round(value, 2),
         "unit": "walker-steps/s",
         "n_gpus": 1,
