@@ -60,6 +60,7 @@ struct afqmc_ctx {
   int* sp_col = nullptr;
   zd* sp_val = nullptr;
   long long sp_nnz = 0;
+  bool sp_sorted = false;  // Spvn columns non-decreasing within every row
   int* spt_rp = nullptr;
   int* spt_col = nullptr;
   zd* spt_val = nullptr;
@@ -387,6 +388,51 @@ ZGemmArgs gargs()
   return a;
 }
 
+// C = alpha * A * B + beta * C for CSR A [nrows][K] and B [K][nw] (walker
+// index last).  Rows averaging 512 entries or more: with columns known to be
+// sorted within rows, 4 to 16 rows share each LDS-staged window of B;
+// otherwise a workgroup of 4 or 16 waves shares each row.  Shorter rows take
+// one wave per (row, 64 walkers), which measured faster than staging.
+int csrmm_n(afqmc_ctx* c, const char* name, int nrows, int K, long long nnz, const int* rp, const int* col,
+            const zd* val, const zd* B, long ldb, zd* C, long ldc, zd alpha, zd beta, bool sorted = false)
+{
+  const int nw = c->nw;
+  const long nchunk = (nw + 63) / 64;
+  const double avg = nrows > 0 ? (double)nnz / nrows : 0.0;
+  if (sorted && avg >= 512) {
+    // rows per workgroup: the most B reuse that still gives 256 workgroups
+    if (((nrows + 15) / 16) * nchunk >= 256)
+      return launch(c, name, [&]() {
+        hipLaunchKernelGGL((csrmm_n_lds_kernel<16, 64>), dim3((unsigned)(((nrows + 15) / 16) * nchunk)), dim3(1024),
+                           0, c->stream, nrows, nw, K, rp, col, val, B, ldb, C, ldc, alpha, beta);
+      });
+    if (((nrows + 7) / 8) * nchunk >= 256)
+      return launch(c, name, [&]() {
+        hipLaunchKernelGGL((csrmm_n_lds_kernel<8, 64>), dim3((unsigned)(((nrows + 7) / 8) * nchunk)), dim3(512), 0,
+                           c->stream, nrows, nw, K, rp, col, val, B, ldb, C, ldc, alpha, beta);
+      });
+    return launch(c, name, [&]() {
+      hipLaunchKernelGGL((csrmm_n_lds_kernel<4, 64>), dim3((unsigned)(((nrows + 3) / 4) * nchunk)), dim3(256), 0,
+                         c->stream, nrows, nw, K, rp, col, val, B, ldb, C, ldc, alpha, beta);
+    });
+  }
+  if (avg >= 4096)
+    return launch(c, name, [&]() {
+      hipLaunchKernelGGL(csrmm_n_split_kernel<16>, dim3((unsigned)(nrows * nchunk)), dim3(1024), 0, c->stream, nrows,
+                         nw, K, rp, col, val, B, ldb, C, ldc, alpha, beta);
+    });
+  if (avg >= 512)
+    return launch(c, name, [&]() {
+      hipLaunchKernelGGL(csrmm_n_split_kernel<4>, dim3((unsigned)(nrows * nchunk)), dim3(256), 0, c->stream, nrows,
+                         nw, K, rp, col, val, B, ldb, C, ldc, alpha, beta);
+    });
+  const long waves = (long)nrows * nchunk;
+  return launch(c, name, [&]() {
+    hipLaunchKernelGGL(csrmm_n_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, c->stream, nrows, nw, K, rp,
+                       col, val, B, ldb, C, ldc, alpha, beta);
+  });
+}
+
 // Batched inverse + determinant of the nw*2 overlap matrices in c->T1.
 int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
 {
@@ -649,13 +695,9 @@ int do_vbias_full(afqmc_ctx* c)
   const int nw = c->nw, K = c->K;
   const long MM = (long)c->M * c->M;
   zd one{1, 0}, zero{0, 0};
-  const long waves = (long)K * ((nw + 63) / 64);
   for (int sp = 0; sp < 2; sp++)
-    RET(launch(c, "csrmm_vbias_t", [&]() {
-      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, K, nw, (int)MM, c->sptt_rp,
-                         c->sptt_col, c->sptt_val, (const zd*)c->Gfull + sp * MM * nw, (long)nw, c->vbias, (long)nw,
-                         one, sp ? one : zero);
-    }));
+    RET(csrmm_n(c, "csrmm_vbias_t", K, (int)MM, c->sp_nnz, c->sptt_rp, c->sptt_col, c->sptt_val,
+                (const zd*)c->Gfull + sp * MM * nw, (long)nw, c->vbias, (long)nw, one, sp ? one : zero, true));
   return AFQMC_OK;
 }
 
@@ -664,11 +706,8 @@ int do_vbias(afqmc_ctx* c)
   const int nw = c->nw, K = c->K, NAK = 2 * c->N * c->M;
   if (c->mode == AFQMC_FP64_CSR) {
     zd one{1, 0}, zero{0, 0};
-    long waves = (long)K * ((nw + 63) / 64);
-    return launch(c, "csrmm_vbias", [&]() {
-      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, K, nw, NAK, c->spt_rp,
-                         c->spt_col, c->spt_val, (const zd*)c->G, (long)nw, c->vbias, (long)nw, one, zero);
-    });
+    return csrmm_n(c, "csrmm_vbias", K, NAK, c->spt_nnz, c->spt_rp, c->spt_col, c->spt_val, (const zd*)c->G,
+                   (long)nw, c->vbias, (long)nw, one, zero, true);
   }
   if (c->mode == AFQMC_FP64_DENSE)
     return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
@@ -709,11 +748,8 @@ int do_vhs(afqmc_ctx* c)
   const long MM = (long)c->M * c->M;
   if (c->mode == AFQMC_FP64_CSR) {
     zd one{1, 0}, zero{0, 0};
-    long waves = MM * ((nw + 63) / 64);
-    RET(launch(c, "csrmm_vhs", [&]() {
-      hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, (int)MM, nw, K, c->sp_rp,
-                         c->sp_col, c->sp_val, (const zd*)c->X, (long)nw, (zd*)c->vHS, (long)nw, one, zero);
-    }));
+    RET(csrmm_n(c, "csrmm_vhs", (int)MM, K, c->sp_nnz, c->sp_rp, c->sp_col, c->sp_val, (const zd*)c->X, (long)nw,
+                (zd*)c->vHS, (long)nw, one, zero, c->sp_sorted));
     dim3 grid((nw + 31) / 32, (unsigned)((MM + 31) / 32));
     return launch(c, "vhs_transpose", [&]() {
       hipLaunchKernelGGL((ztranspose_kernel<zd, zd>), grid, dim3(256), 0, c->stream, (const zd*)c->vHS, c->vHSw,
@@ -1158,11 +1194,8 @@ int do_energy(afqmc_ctx* c, double* eav_out)
   if (c->mode == AFQMC_FP64_CSR) {
     if (c->v_rp) {
       zd one{1, 0}, zero{0, 0};
-      long waves = NAK * ((nw + 63) / 64);
-      RET(launch(c, "csrmm_vakbl", [&]() {
-        hipLaunchKernelGGL(csrmm_n_kernel, dim3((waves + 3) / 4), dim3(256), 0, c->stream, (int)NAK, nw, (int)NAK,
-                           c->v_rp, c->v_col, c->v_val, (const zd*)c->G, (long)nw, c->Gcloc, (long)nw, one, zero);
-      }));
+      RET(csrmm_n(c, "csrmm_vakbl", (int)NAK, (int)NAK, c->v_nnz, c->v_rp, c->v_col, c->v_val, (const zd*)c->G,
+                  (long)nw, c->Gcloc, (long)nw, one, zero));
       // E2 = sum G.(Vakbl G) lands in the direct slot (exchange slot zero),
       // so E_L = E2/2 + E1 as energy.hpp:93-100
       RET(launch(c, "energy_two", [&]() {
@@ -1491,8 +1524,14 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
   if (spvn->nrows != MM || spvn->ncols != K || spvn->nnz < 0 || !spvn->rowptr || (spvn->nnz > 0 && (!spvn->col || !spvn->val)))
     return fail(c, AFQMC_ERR_SHAPE, "Spvn must be CSR [M*M][K]");
   if (spvn->rowptr[MM] - spvn->rowptr[0] != spvn->nnz) return fail(c, AFQMC_ERR_SHAPE, "Spvn rowptr/nnz mismatch");
-  if (vakbl && vakbl->nnz > 0 && (vakbl->nrows != 2 * NM || vakbl->ncols != 2 * NM))
+  if (vakbl && vakbl->nnz > 0 &&
+      (vakbl->nrows != 2 * NM || vakbl->ncols != 2 * NM || !vakbl->rowptr || !vakbl->col || !vakbl->val))
     return fail(c, AFQMC_ERR_SHAPE, "Vakbl must be CSR [2NM][2NM]");
+  if (vakbl && vakbl->nnz > 0 && vakbl->rowptr[2 * NM] - vakbl->rowptr[0] != vakbl->nnz)
+    return fail(c, AFQMC_ERR_SHAPE, "Vakbl rowptr/nnz mismatch");
+  // entries of row r sit at rowptr[r] - rowptr[0] from these bases
+  const int* scol = spvn->nnz > 0 ? spvn->col + spvn->rowptr[0] : nullptr;
+  const double* sval = spvn->nnz > 0 ? spvn->val + 2 * (long)spvn->rowptr[0] : nullptr;
   if (dt <= 0.0) return fail(c, AFQMC_ERR_ARG, "dt must be positive");
   if (c->have_system) return fail(c, AFQMC_ERR_STATE, "system already uploaded");
   c->dt_sys = dt;
@@ -1527,12 +1566,21 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     std::vector<int> rp0(MM + 1);
     for (long i = 0; i <= MM; i++) rp0[i] = spvn->rowptr[i] - spvn->rowptr[0];
     HIPCHK(c, hipMemcpyAsync(rp, rp0.data(), sizeof(int) * (MM + 1), hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(col, spvn->col, sizeof(int) * nnz, hipMemcpyHostToDevice, st));
-    HIPCHK(c, hipMemcpyAsync(val, spvn->val, sizeof(zd) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(col, scol, sizeof(int) * nnz, hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(val, sval, sizeof(zd) * nnz, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipStreamSynchronize(st));
   }
-  for (long p = 0; p < nnz; p += std::max<long>(1, nnz / 64))
-    if (spvn->col[p] < 0 || spvn->col[p] >= K) return fail(c, AFQMC_ERR_SHAPE, "Spvn column index out of range");
+  for (long r = 0; r < MM; r++)
+    if (spvn->rowptr[r + 1] < spvn->rowptr[r]) return fail(c, AFQMC_ERR_SHAPE, "Spvn rowptr is not monotone");
+  for (long p = 0; p < nnz; p++)
+    if (scol[p] < 0 || scol[p] >= K) return fail(c, AFQMC_ERR_SHAPE, "Spvn column index out of range");
+  c->sp_sorted = true;
+  for (long r = 0; r < MM && c->sp_sorted; r++)
+    for (long q = spvn->rowptr[r] - spvn->rowptr[0] + 1; q < spvn->rowptr[r + 1] - spvn->rowptr[0]; q++)
+      if (scol[q] < scol[q - 1]) {
+        c->sp_sorted = false;
+        break;
+      }
   // dense Ld[n][ik] (FP64), then the half rotation Lt[s][n][a][k] = sum_i trial_s(i,a) Ld[n][i][k]
   zd* Ld64 = nullptr;
   RET(dalloc(c, &Ld64, K * MM));
@@ -1571,14 +1619,14 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       std::vector<int> trp(K + 1, 0), tcol(nnz);
       std::vector<zd> tval(nnz);
       const int r0 = spvn->rowptr[0];
-      for (long q = 0; q < nnz; q++) trp[spvn->col[q] + 1]++;
+      for (long q = 0; q < nnz; q++) trp[scol[q] + 1]++;
       for (long n = 0; n < K; n++) trp[n + 1] += trp[n];
       std::vector<int> pos(trp.begin(), trp.end() - 1);
       for (long r = 0; r < MM; r++)
         for (int q = spvn->rowptr[r] - r0; q < spvn->rowptr[r + 1] - r0; q++) {
-          const int d = pos[spvn->col[q]]++;
+          const int d = pos[scol[q]]++;
           tcol[d] = (int)r;
-          tval[d] = zd{spvn->val[2 * q], spvn->val[2 * q + 1]};
+          tval[d] = zd{sval[2 * q], sval[2 * q + 1]};
         }
       RET(dalloc(c, &c->sptt_rp, K + 1));
       RET(dalloc(c, &c->sptt_col, nnz));
@@ -1618,14 +1666,20 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     dfree(c, Ld64);
     if (vakbl && vakbl->nnz > 0) {
       const long vn = vakbl->nnz, vr = vakbl->nrows;
+      const int* vcol = vakbl->col + vakbl->rowptr[0];
+      for (long r = 0; r < vr; r++)
+        if (vakbl->rowptr[r + 1] < vakbl->rowptr[r]) return fail(c, AFQMC_ERR_SHAPE, "Vakbl rowptr is not monotone");
+      for (long q = 0; q < vn; q++)
+        if (vcol[q] < 0 || vcol[q] >= vr) return fail(c, AFQMC_ERR_SHAPE, "Vakbl column index out of range");
       RET(dalloc(c, &c->v_rp, vr + 1));
       RET(dalloc(c, &c->v_col, vn));
       RET(dalloc(c, &c->v_val, vn));
       std::vector<int> rp0(vr + 1);
       for (long i = 0; i <= vr; i++) rp0[i] = vakbl->rowptr[i] - vakbl->rowptr[0];
       HIPCHK(c, hipMemcpyAsync(c->v_rp, rp0.data(), sizeof(int) * (vr + 1), hipMemcpyHostToDevice, st));
-      HIPCHK(c, hipMemcpyAsync(c->v_col, vakbl->col, sizeof(int) * vn, hipMemcpyHostToDevice, st));
-      HIPCHK(c, hipMemcpyAsync(c->v_val, vakbl->val, sizeof(zd) * vn, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->v_col, vakbl->col + vakbl->rowptr[0], sizeof(int) * vn, hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->v_val, vakbl->val + 2 * (long)vakbl->rowptr[0], sizeof(zd) * vn, hipMemcpyHostToDevice, st));
+      c->v_nnz = vn;
       RET(dalloc(c, &c->Gcloc, 2 * NM * c->nw));
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Lt64);

@@ -194,6 +194,147 @@ __global__ __launch_bounds__(256) void csrmm_n_kernel(int nrows, int nw, int K, 
   }
 }
 
+// csrmm 'N' for long rows: one workgroup of S waves per (row, 64-walker
+// chunk).  Wave s sums a contiguous segment of the row's entries into two
+// accumulators (even and odd entries, so consecutive FMAs do not wait on
+// each other); the S partial sums are reduced through LDS in a fixed order.
+// Same semantics as csrmm_n_kernel (entries with col >= K skipped).
+template <int S>
+__global__ __launch_bounds__(64 * S) void csrmm_n_split_kernel(int nrows, int nw, int K, const int* __restrict__ rowptr,
+                                                              const int* __restrict__ col,
+                                                              const zd* __restrict__ val, const zd* __restrict__ B,
+                                                              long ldb, zd* __restrict__ C, long ldc, zd alpha,
+                                                              zd beta)
+{
+  __shared__ zd part[S][64];
+  const int nchunk = (nw + 63) / 64;
+  const int r = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+  const int sw = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = ch * 64 + lane;
+  const int p0 = rowptr[0];
+  const int pb = rowptr[r] - p0, pe = rowptr[r + 1] - p0;
+  const int len = pe - pb, per = (len + S - 1) / S;
+  const int qb = pb + min(len, sw * per), qe = pb + min(len, (sw + 1) * per);
+  zd a0{0, 0}, a1{0, 0};
+  if (w < nw) {
+    int p = qb;
+    for (; p + 1 < qe; p += 2) {
+      const int c0 = col[p], c1 = col[p + 1];
+      const zd v0 = val[p], v1 = val[p + 1];
+      if (c0 < K) {
+        const zd b0 = B[(long)c0 * ldb + w];
+        a0.re += v0.re * b0.re - v0.im * b0.im;
+        a0.im += v0.re * b0.im + v0.im * b0.re;
+      }
+      if (c1 < K) {
+        const zd b1 = B[(long)c1 * ldb + w];
+        a1.re += v1.re * b1.re - v1.im * b1.im;
+        a1.im += v1.re * b1.im + v1.im * b1.re;
+      }
+    }
+    if (p < qe) {
+      const int c0 = col[p];
+      const zd v0 = val[p];
+      if (c0 < K) {
+        const zd b0 = B[(long)c0 * ldb + w];
+        a0.re += v0.re * b0.re - v0.im * b0.im;
+        a0.im += v0.re * b0.im + v0.im * b0.re;
+      }
+    }
+  }
+  part[sw][lane] = zadd(a0, a1);
+  __syncthreads();
+  if (sw == 0 && w < nw) {
+    zd acc = part[0][lane];
+#pragma unroll
+    for (int q = 1; q < S; q++) acc = zadd(acc, part[q][lane]);
+    zd o = zmul(alpha, acc);
+    if (beta.re != 0.0 || beta.im != 0.0) o = zadd(o, zmul(beta, C[(long)r * ldc + w]));
+    C[(long)r * ldc + w] = o;
+  }
+}
+
+// value of lane j (j the same in every lane) without going through LDS
+__device__ inline int rdlane(int x, int j) { return __builtin_amdgcn_readlane(x, j); }
+__device__ inline double rdlane(double x, int j)
+{
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffLL), j);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), j);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// csrmm 'N' for long rows whose columns are sorted within each row: a
+// workgroup of R waves takes R consecutive rows and one chunk of 64 walkers,
+// and walks the columns in windows of CB.  Each window of B (CB rows x 64
+// walkers) is staged in LDS once and read by all R rows, so B is fetched
+// once per R rows instead of once per row.  Entries with col >= K are never
+// reached (the reference's csrmm skips them too).
+template <int R, int CB>
+__global__ __launch_bounds__(64 * R) void csrmm_n_lds_kernel(int nrows, int nw, int K, const int* __restrict__ rowptr,
+                                                            const int* __restrict__ col, const zd* __restrict__ val,
+                                                            const zd* __restrict__ B, long ldb, zd* __restrict__ C,
+                                                            long ldc, zd alpha, zd beta)
+{
+  __shared__ zd Bs[CB][64];
+  const int nchunk = (nw + 63) / 64;
+  const int rb = blockIdx.x / nchunk, ch = blockIdx.x % nchunk;
+  const int sw = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = rb * R + sw;
+  const int w = ch * 64 + lane;
+  const int p0 = rowptr[0];
+  int p = 0, pe = 0;
+  if (r < nrows) {
+    p = rowptr[r] - p0;
+    pe = rowptr[r + 1] - p0;
+  }
+  p = __builtin_amdgcn_readfirstlane(p);  // one row per wave
+  pe = __builtin_amdgcn_readfirstlane(pe);
+  zd a0{0, 0}, a1{0, 0};
+  for (int c0 = 0; c0 < K; c0 += CB) {
+    for (int e = threadIdx.x; e < CB * 64; e += 64 * R) {
+      const int cc = e >> 6, l = e & 63, ww = ch * 64 + l;
+      Bs[cc][l] = (c0 + cc < K && ww < nw) ? B[(long)(c0 + cc) * ldb + ww] : zd{0, 0};
+    }
+    __syncthreads();
+    const int cend = c0 + CB;
+    // the row's entries in this window, 64 at a time: lane i loads entry
+    // p + i, the window's entries form a prefix (columns sorted), and each
+    // is read from its lane into scalar registers
+    for (;;) {
+      const int q = p + lane;
+      const bool inw = q < pe && col[q] < cend;
+      const int cq = inw ? col[q] - c0 : 0;
+      const zd vq = inw ? val[q] : zd{0, 0};
+      const int n = __popcll(__ballot(inw));
+      for (int j = 0; j + 1 < n; j += 2) {
+        const int c_0 = rdlane(cq, j), c_1 = rdlane(cq, j + 1);
+        const zd v0{rdlane(vq.re, j), rdlane(vq.im, j)}, v1{rdlane(vq.re, j + 1), rdlane(vq.im, j + 1)};
+        const zd b0 = Bs[c_0][lane], b1 = Bs[c_1][lane];
+        a0.re += v0.re * b0.re - v0.im * b0.im;
+        a0.im += v0.re * b0.im + v0.im * b0.re;
+        a1.re += v1.re * b1.re - v1.im * b1.im;
+        a1.im += v1.re * b1.im + v1.im * b1.re;
+      }
+      if (n & 1) {
+        const int c_0 = rdlane(cq, n - 1);
+        const zd v0{rdlane(vq.re, n - 1), rdlane(vq.im, n - 1)};
+        const zd b0 = Bs[c_0][lane];
+        a0.re += v0.re * b0.re - v0.im * b0.im;
+        a0.im += v0.re * b0.im + v0.im * b0.re;
+      }
+      p += n;
+      if (n < 64) break;
+    }
+    __syncthreads();
+  }
+  if (r < nrows && w < nw) {
+    zd o = zmul(alpha, zadd(a0, a1));
+    if (beta.re != 0.0 || beta.im != 0.0) o = zadd(o, zmul(beta, C[(long)r * ldc + w]));
+    C[(long)r * ldc + w] = o;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tiled transpose of complex matrices: out[c][r] = in[r][c], in is R x Cc.
 // Output precision OT (double or float pairs).

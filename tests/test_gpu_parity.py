@@ -482,3 +482,19 @@ def test_run_ragged_shapes(require_gpu, M, N, K, nw, mode):
         close(g["eav"], o["eav"], 1e-9, "Eav trajectory")
         close(g["W"], o["W"], 1e-9, "final walkers")
         close(g["Wd"][:, :5], o["Wd"][:, :5], 1e-9, "final W_data")
+
+
+@pytest.mark.parametrize("bad", ["column", "rowptr"])
+def test_upload_rejects_malformed_spvn(require_gpu, bad):
+    """A Cholesky CSR with an out-of-range column or a decreasing row pointer
+    is refused at upload (AFQMC_ERR_SHAPE) before any kernel reads it."""
+    s = m.make_system(12, 3, 8, density=0.5, sigma_L=0.03, seed=5)
+    col, rp = s.spvn.col.copy(), s.spvn.rowptr.copy()
+    if bad == "column":
+        col[len(col) // 2] = s.nchol
+    else:
+        rp[5] = rp[6] + 1  # row 5 would end before it starts
+    s.spvn = m.Csr(s.spvn.nrows, s.spvn.ncols, rp, col, s.spvn.val)
+    with pytest.raises(nat.AfqmcError) as ei:
+        m.AfqmcSys(s, 2, "fp64_csr")
+    assert ei.value.code == 2
