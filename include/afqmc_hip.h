@@ -115,7 +115,13 @@ typedef enum {
    * added in FP64, which bounds the FP32 accumulation error.  0: one pass
    * over all columns.  May be changed at any time.  No counterpart in the
    * reference (its energy is FP64 throughout). */
-  AFQMC_OPT_EXCHANGE_CHUNK = 4
+  AFQMC_OPT_EXCHANGE_CHUNK = 4,
+  /* AFQMC_MIXED_BF16 with M a multiple of 64 up to 512 and N a multiple of
+   * 16: 1 runs the whole propagator (P W, the six Taylor terms, P U) as one
+   * kernel per walker and 32 columns, U kept in LDS; 0 (default) runs it as
+   * eight GEMM launches, which measured 2 % faster at 512 orbitals.  Both
+   * give the same walkers bit for bit.  May be changed at any time. */
+  AFQMC_OPT_FUSED_PROPAGATOR = 5
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

@@ -156,6 +156,10 @@ class AfqmcSys:
         """Always use the blocked Householder kernel for the orthogonalisation."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ORTHO_HOUSEHOLDER, 1.0 if on else 0.0))
 
+    def set_fused_propagator(self, on):
+        """Split-BF16 mode: the whole propagator in one kernel (1) or as eight GEMMs (0)."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_FUSED_PROPAGATOR, 1.0 if on else 0.0))
+
     def set_exchange_chunk(self, cols):
         """Dense exchange: accumulate the exchange GEMM over chunks of `cols`
         columns, adding the chunks' energies in FP64 (0: one pass)."""

@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "gemm.hpp"
 #include "kernels.hpp"
+#include "propagate.hpp"
 
 #include <dlfcn.h>
 
@@ -126,6 +127,7 @@ struct afqmc_ctx {
   size_t vws_bytes = 0;
   // orthogonalisation: Cholesky QR scratch (allocated at the first call)
   bool ortho_householder = false;  // always use the blocked Householder kernel
+  bool fused_prop = false;         // split-BF16 propagator in one kernel when the shapes allow
   long exch_chunk = 2048;          // dense exchange: columns per accumulation chunk (0: one pass)
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
   zd* oX = nullptr;   // [2][2nw][M][N] the two Cholesky QR passes
@@ -840,10 +842,40 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
   return AFQMC_OK;
 }
 
+// W <- P E6(V) P W in one kernel per (walker, 32 columns) (propagate.hpp):
+// split-BF16 mode, M a multiple of 64 up to 512, N a multiple of 16.
+bool fused_prop_ok(const afqmc_ctx* c)
+{
+  return c->fused_prop && c->mode == AFQMC_MIXED_BF16 && c->bsplit && c->M % 64 == 0 && c->M <= PF_MMAX &&
+         c->N % 16 == 0;
+}
+
+int propagate_fused(afqmc_ctx* c)
+{
+  PropFusedArgs a;
+  a.W = c->W;
+  a.S = (float*)c->S;
+  a.P = (const zb*)c->propg32;
+  a.V = (const zb*)c->vHS;
+  a.M = c->M;
+  a.N = c->N;
+  a.nw = c->nw;
+  a.ncb = 2 * c->N / PF_NB;
+  const unsigned grid = (unsigned)((long)c->nw * a.ncb);
+  // 16 waves of 16 or 32 rows each
+  return launch(c, "prop_fused", [&]() {
+    if (c->M <= 256)
+      hipLaunchKernelGGL((prop_fused_kernel<1, 16>), dim3(grid), dim3(1024), 0, c->stream, a);
+    else
+      hipLaunchKernelGGL((prop_fused_kernel<2, 16>), dim3(grid), dim3(1024), 0, c->stream, a);
+  });
+}
+
 int do_propagate(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
   c->g_fresh = false;
+  if (fused_prop_ok(c)) return propagate_fused(c);
   if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHSw);
   if (c->mode == AFQMC_FP64_DENSE)
     return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHS);
@@ -1504,6 +1536,11 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     if (value < 0 || value != (long)value || ((long)value % 256) != 0)
       return fail(c, AFQMC_ERR_ARG, "exchange chunk must be 0 or a positive multiple of 256");
     c->exch_chunk = (long)value;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_FUSED_PROPAGATOR) {
+    drop_graphs(c);
+    c->fused_prop = value != 0.0;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {

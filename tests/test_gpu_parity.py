@@ -498,3 +498,27 @@ def test_upload_rejects_malformed_spvn(require_gpu, bad):
     with pytest.raises(nat.AfqmcError) as ei:
         m.AfqmcSys(s, 2, "fp64_csr")
     assert ei.value.code == 2
+
+
+@pytest.mark.parametrize("shape", ["N16_M64", "N64_M256", "N128_M512"])
+def test_fused_propagator_matches_gemm_chain(require_gpu, shape):
+    """The one-kernel split-BF16 propagator gives the walkers of the eight-GEMM
+    chain bit for bit (same operand splits, MFMA order and FP64 epilogues)."""
+    s, p = m.preset_system(shape)
+    nw = 16
+    W = perturbed_walkers(s, nw, seed=41, noise=0.05)
+    x = m.fields(p["seed"], s.nchol, nw, 0)
+    out = []
+    for fused in (False, True):
+        say(f"propagator {shape} fused={fused}")
+        g = m.AfqmcSys(s, nw, "mixed_bf16", dense_exchange=False)
+        g.set_fused_propagator(fused)
+        g.set(nat.BUF_W, W)
+        g.calculate_mixed_density_matrix()
+        g.get_vbias()
+        g.fields(x)
+        g.get_vHS()
+        g.propagate()
+        out.append(g.W)
+        g.close()
+    assert np.array_equal(out[0], out[1]), f"max diff {np.abs(out[0] - out[1]).max():.3e}"
