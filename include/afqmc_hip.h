@@ -240,6 +240,24 @@ int afqmc_tune(afqmc_ctx* ctx, double dt);
  * and replay them from then on in afqmc_substep/afqmc_run (enable != 0). */
 int afqmc_use_graphs(afqmc_ctx* ctx, int enable);
 
+/* Several independent systems (contexts of any sizes and modes on the same
+ * GPU) advanced together.  No counterpart in the reference, which runs one
+ * system; for batches of uneven systems.  afqmc_group_substep captures one
+ * substep of every member into a single HIP graph on first use, each member
+ * on its own stream branch, and replays it: one graph launch per substep
+ * advances every system, their kernels running side by side.  Each member
+ * needs its fields staged (afqmc_stage_fields) for the substep; a member's
+ * results equal those of afqmc_substep on that member alone.  The graph is
+ * captured again when dt changes or a member's staged fields or options
+ * change.  Members stay usable on their own (their streams wait for the
+ * group's substeps); afqmc_group_sync waits for everything. */
+typedef struct afqmc_group afqmc_group;
+int afqmc_group_create(afqmc_ctx* const* ctxs, int n, afqmc_group** out);
+int afqmc_group_destroy(afqmc_group* group);
+int afqmc_group_substep(afqmc_group* group, long substep, int do_ortho, double dt);
+int afqmc_group_sync(afqmc_group* group);
+const char* afqmc_group_error(const afqmc_group* group);
+
 /* ---- measurement -------------------------------------------------------- */
 /* Per-kernel timing with HIP events on the context's stream.  While enabled,
  * every launch is bracketed by events; afqmc_kernel_stats fills up to `cap`

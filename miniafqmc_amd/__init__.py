@@ -1,4 +1,4 @@
 """MI355X walker step of miniAFQMC: C-ABI library libafqmc_hip.so plus host mirrors."""
 from ._native import AfqmcError, MODES, lib  # noqa: F401  (raises if the library is not built)
-from .afqmc import AfqmcSys  # noqa: F401
+from .afqmc import AfqmcGroup, AfqmcSys  # noqa: F401
 from .system import PRESETS, System, Csr, fields, fields_block, make_system, preset_system  # noqa: F401

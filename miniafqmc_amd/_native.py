@@ -38,6 +38,7 @@ EXPORTED = [
     "afqmc_weights", "afqmc_branch", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
     "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
     "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats", "afqmc_zgemm",
+    "afqmc_group_create", "afqmc_group_destroy", "afqmc_group_substep", "afqmc_group_sync", "afqmc_group_error",
     "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
 ]
 
@@ -121,6 +122,11 @@ def _load():
         "afqmc_run": (I, [P, L, I, I, I, C.c_double, D]),
         "afqmc_tune": (I, [P, C.c_double]),
         "afqmc_use_graphs": (I, [P, I]),
+        "afqmc_group_create": (I, [C.POINTER(C.c_void_p), I, C.POINTER(C.c_void_p)]),
+        "afqmc_group_destroy": (I, [P]),
+        "afqmc_group_substep": (I, [P, C.c_long, I, C.c_double]),
+        "afqmc_group_sync": (I, [P]),
+        "afqmc_group_error": (C.c_char_p, [P]),
         "afqmc_profile": (I, [P, I]),
         "afqmc_kernel_stats": (I, [P, I, P, C.POINTER(C.c_longlong), D, C.POINTER(C.c_int)]),
         "afqmc_zgemm": (I, [I, C.c_char, C.c_char, I, I, I, D, D, I, D, I, D, D, I]),

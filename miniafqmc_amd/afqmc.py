@@ -234,3 +234,36 @@ class AfqmcSys:
         cnt = C.c_int(0)
         self._check(nat.lib.afqmc_kernel_stats(self.ctx, cap, C.cast(names, C.c_void_p), n, ms, C.byref(cnt)))
         return {names[i].value.decode(): (int(n[i]), float(ms[i])) for i in range(min(cnt.value, cap))}
+
+
+class AfqmcGroup:
+    """Several AfqmcSys (independent systems, any sizes) advanced together:
+    one HIP graph launch per substep (afqmc_group_substep)."""
+
+    def __init__(self, systems):
+        self.members = list(systems)
+        arr = (C.c_void_p * len(self.members))(*[g.ctx.value for g in self.members])
+        h = C.c_void_p()
+        nat.check(nat.lib.afqmc_group_create(arr, len(self.members), C.byref(h)))
+        self.h = h
+
+    def _check(self, code):
+        if code != 0:
+            raise nat.AfqmcError(code, nat.lib.afqmc_group_error(self.h).decode())
+
+    def substep(self, s, do_ortho, dt):
+        self._check(nat.lib.afqmc_group_substep(self.h, int(s), 1 if do_ortho else 0, float(dt)))
+
+    def sync(self):
+        self._check(nat.lib.afqmc_group_sync(self.h))
+
+    def close(self):
+        if self.h:
+            nat.lib.afqmc_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -128,6 +128,7 @@ struct afqmc_ctx {
   // orthogonalisation: Cholesky QR scratch (allocated at the first call)
   bool ortho_householder = false;  // always use the blocked Householder kernel
   bool fused_prop = false;         // split-BF16 propagator in one kernel when the shapes allow
+  long graph_gen = 0;              // bumped whenever captured substeps become stale
   long exch_chunk = 2048;          // dense exchange: columns per accumulation chunk (0: one pass)
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
   zd* oX = nullptr;   // [2][2nw][M][N] the two Cholesky QR passes
@@ -1364,6 +1365,7 @@ void htranspose(const zd* in, zd* out, long R, long Cc)
 // Drop the captured graphs (the staged block, dt or a buffer changed).
 void drop_graphs(afqmc_ctx* c)
 {
+  c->graph_gen++;  // groups holding this context's substep capture again
   for (int i = 0; i < 2; i++) {
     if (c->gexec[i]) (void)hipGraphExecDestroy(c->gexec[i]);
     if (c->graph[i]) (void)hipGraphDestroy(c->graph[i]);
@@ -1377,7 +1379,29 @@ void drop_graphs(afqmc_ctx* c)
 // here, outside the capture.  The graph starts from the inverses of the
 // current walkers' overlap matrices (the state every substep ends in), so
 // its density matrix reuses them.
+int capture_prep(afqmc_ctx* c, int ortho);
+
 int capture_substep(afqmc_ctx* c, int ortho, double dt)
+{
+  RET(capture_prep(c, ortho));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r != AFQMC_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return r;
+  }
+  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  c->graph[ortho] = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec[ortho], g, nullptr, nullptr, 0));
+  return AFQMC_OK;
+}
+
+// Everything a captured substep needs that it would otherwise allocate or
+// compute on first use, done outside the capture; the context's stream is
+// idle afterwards.
+int capture_prep(afqmc_ctx* c, int ortho)
 {
   const long MN = (long)c->M * c->N, nb = 2L * c->nw;
   if (!c->d_sub) RET(dalloc(c, &c->d_sub, 1));
@@ -1398,17 +1422,6 @@ int capture_substep(afqmc_ctx* c, int ortho, double dt)
   }
   if (!c->t1inv_valid) RET(do_overlaps(c));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
-  hipGraph_t g = nullptr;
-  const hipError_t e = hipStreamEndCapture(c->stream, &g);
-  if (r != AFQMC_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return r;
-  }
-  if (e != hipSuccess) return fail(c, AFQMC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-  c->graph[ortho] = g;
-  HIPCHK(c, hipGraphInstantiate(&c->gexec[ortho], g, nullptr, nullptr, 0));
   return AFQMC_OK;
 }
 
@@ -2214,3 +2227,184 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
   c->stream = nullptr;
   return r;
 }
+
+// ---------------------------------------------------------------------------
+// Several contexts advanced together: one HIP graph holds a substep of every
+// member, each on its member's own stream (forked from and joined to the
+// group's stream inside the capture), so one graph launch advances all the
+// systems and their kernels run side by side.
+struct afqmc_group {
+  std::vector<afqmc_ctx*> ctx;
+  std::vector<long> gen[2];  // members' graph generations at capture
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double g_dt = 0.0;
+  std::string err;
+};
+
+namespace {
+
+int gfail(afqmc_group* g, int code, const std::string& m)
+{
+  g->err = m;
+  return code;
+}
+
+void group_drop(afqmc_group* g)
+{
+  for (int i = 0; i < 2; i++) {
+    if (g->gexec[i]) (void)hipGraphExecDestroy(g->gexec[i]);
+    if (g->graph[i]) (void)hipGraphDestroy(g->graph[i]);
+    g->gexec[i] = nullptr;
+    g->graph[i] = nullptr;
+    g->gen[i].clear();
+  }
+}
+
+int group_capture(afqmc_group* g, int ortho, double dt)
+{
+  for (afqmc_ctx* c : g->ctx) {
+    const int r = capture_prep(c, ortho);
+    if (r != AFQMC_OK) return gfail(g, r, std::string("member: ") + c->err);
+  }
+  if (hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return gfail(g, AFQMC_ERR_HIP, "group capture did not start");
+  int r = hipEventRecord(g->fork, g->stream) == hipSuccess ? AFQMC_OK : AFQMC_ERR_HIP;
+  for (size_t i = 0; i < g->ctx.size() && r == AFQMC_OK; i++) {
+    afqmc_ctx* c = g->ctx[i];
+    if (hipStreamWaitEvent(c->stream, g->fork, 0) != hipSuccess) {
+      r = AFQMC_ERR_HIP;
+      break;
+    }
+    r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
+    if (r != AFQMC_OK) g->err = std::string("member: ") + c->err;
+    if (r == AFQMC_OK && (hipEventRecord(g->join[i], c->stream) != hipSuccess ||
+                          hipStreamWaitEvent(g->stream, g->join[i], 0) != hipSuccess))
+      r = AFQMC_ERR_HIP;
+  }
+  hipGraph_t gr = nullptr;
+  const hipError_t e = hipStreamEndCapture(g->stream, &gr);
+  if (r != AFQMC_OK || e != hipSuccess) {
+    if (gr) (void)hipGraphDestroy(gr);
+    return r != AFQMC_OK ? r : gfail(g, AFQMC_ERR_HIP, std::string("group capture: ") + hipGetErrorString(e));
+  }
+  g->graph[ortho] = gr;
+  if (hipGraphInstantiate(&g->gexec[ortho], gr, nullptr, nullptr, 0) != hipSuccess)
+    return gfail(g, AFQMC_ERR_HIP, "group graph instantiation failed");
+  g->gen[ortho].clear();
+  for (afqmc_ctx* c : g->ctx) g->gen[ortho].push_back(c->graph_gen);
+  return AFQMC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int afqmc_group_create(afqmc_ctx* const* ctxs, int n, afqmc_group** out)
+{
+  if (!out) return AFQMC_ERR_ARG;
+  *out = nullptr;
+  if (!ctxs || n <= 0) return AFQMC_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!ctxs[i] || !ctxs[i]->have_system) return AFQMC_ERR_STATE;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return AFQMC_ERR_ARG;
+  }
+  afqmc_group* g = new afqmc_group;
+  g->ctx.assign(ctxs, ctxs + n);
+  g->join.assign(n, nullptr);
+  bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&g->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&g->done, hipEventDisableTiming) == hipSuccess;
+  for (int i = 0; i < n && ok; i++) ok = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    afqmc_group_destroy(g);
+    return AFQMC_ERR_HIP;
+  }
+  *out = g;
+  return AFQMC_OK;
+}
+
+int afqmc_group_destroy(afqmc_group* g)
+{
+  if (!g) return AFQMC_OK;
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  group_drop(g);
+  for (hipEvent_t e : g->join)
+    if (e) (void)hipEventDestroy(e);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->done) (void)hipEventDestroy(g->done);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return AFQMC_OK;
+}
+
+const char* afqmc_group_error(const afqmc_group* g) { return g ? g->err.c_str() : "null group"; }
+
+int afqmc_group_substep(afqmc_group* g, long substep, int do_ortho, double dt)
+{
+  if (!g) return AFQMC_ERR_ARG;
+  const int o = do_ortho ? 1 : 0;
+  for (afqmc_ctx* c : g->ctx)
+    if (!c->xstage || substep < 0 || substep >= c->nstage)
+      return gfail(g, AFQMC_ERR_STATE, "a member has no staged fields for this substep");
+  // profiling and tuning run the members' substeps eagerly, one by one
+  bool eager = false;
+  for (afqmc_ctx* c : g->ctx) eager |= c->prof || c->tuning;
+  if (eager) {
+    for (afqmc_ctx* c : g->ctx) {
+      const int r = afqmc_substep(c, substep, do_ortho, dt);
+      if (r != AFQMC_OK) return gfail(g, r, std::string("member: ") + c->err);
+    }
+    return AFQMC_OK;
+  }
+  if (g->g_dt != dt) {
+    group_drop(g);
+    g->g_dt = dt;
+  }
+  bool stale = !g->gexec[o];
+  for (size_t i = 0; i < g->ctx.size() && !stale; i++) stale = g->gen[o][i] != g->ctx[i]->graph_gen;
+  if (stale) {
+    if (g->gexec[o]) (void)hipGraphExecDestroy(g->gexec[o]);
+    if (g->graph[o]) (void)hipGraphDestroy(g->graph[o]);
+    g->gexec[o] = nullptr;
+    g->graph[o] = nullptr;
+    const int r = group_capture(g, o, dt);
+    if (r != AFQMC_OK) return r;
+  }
+  // the graph starts from the inverses of the members' current walkers, and
+  // from work already queued on the members' streams
+  for (size_t i = 0; i < g->ctx.size(); i++) {
+    afqmc_ctx* c = g->ctx[i];
+    if (!c->t1inv_valid) {
+      const int r = do_overlaps(c);
+      if (r != AFQMC_OK) return gfail(g, r, std::string("member: ") + c->err);
+    }
+    if (hipEventRecord(g->join[i], c->stream) != hipSuccess || hipStreamWaitEvent(g->stream, g->join[i], 0) != hipSuccess)
+      return gfail(g, AFQMC_ERR_HIP, "ordering the members' streams");
+    hipLaunchKernelGGL(set_long_kernel, dim3(1), dim3(64), 0, g->stream, c->d_sub, substep);
+  }
+  if (hipGraphLaunch(g->gexec[o], g->stream) != hipSuccess || hipEventRecord(g->done, g->stream) != hipSuccess)
+    return gfail(g, AFQMC_ERR_HIP, "group graph launch failed");
+  // later work on a member's own stream waits for the group's substep
+  for (afqmc_ctx* c : g->ctx) {
+    if (hipStreamWaitEvent(c->stream, g->done, 0) != hipSuccess) return gfail(g, AFQMC_ERR_HIP, "ordering the members");
+    c->t1inv_valid = true;
+    c->g_fresh = false;
+  }
+  return AFQMC_OK;
+}
+
+int afqmc_group_sync(afqmc_group* g)
+{
+  if (!g) return AFQMC_ERR_ARG;
+  if (hipStreamSynchronize(g->stream) != hipSuccess) return gfail(g, AFQMC_ERR_HIP, "group stream");
+  for (afqmc_ctx* c : g->ctx)
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfail(g, AFQMC_ERR_HIP, "member stream");
+  return AFQMC_OK;
+}
+
+} // extern "C"
