@@ -77,6 +77,85 @@ def kernel_flops(name, M, N, K, nw):
     return None if c is None else 8.0 * c
 
 
+def bench_multi(args, m):
+    """Uneven batch of independent systems (M = 96 ... 512) advanced together,
+    one graph launch per substep, against the same systems run one after
+    another.  value = walkers x substeps summed over the systems / time."""
+    t0 = time.time()
+    systems = m.multi_systems()
+    print(f"[bench] {len(systems)} systems generated in {time.time() - t0:.1f}s", flush=True)
+    nsub = args.warmup + args.steps
+    spe = args.substeps_per_energy or args.steps
+    northo = args.northo or 10
+
+    def contexts():
+        out = []
+        for s, nw, seed in systems:
+            g = m.AfqmcSys(s, nw, args.mode, energy_precision=args.energy_level)
+            g.set_exchange_chunk(args.exchange_chunk)
+            g.stage_fields(m.fields_block(seed, s.nchol, nw, 0, nsub))
+            if args.tune:
+                g.tune(s.dt)
+            g.init_walkers()
+            g.calculate_mixed_density_matrix()
+            g.calculate_energy()
+            out.append(g)
+        return out
+
+    def block(step_fn, ctxs):
+        for t in range(args.warmup):
+            step_fn(t)
+        for g in ctxs:
+            g.sync()
+        ta = time.perf_counter()
+        for t in range(args.warmup, nsub):
+            step_fn(t)
+            if (t - args.warmup + 1) % spe == 0:
+                for g in ctxs:
+                    g.calculate_mixed_density_matrix()
+                    g.calculate_energy()
+        for g in ctxs:
+            g.sync()
+        return time.perf_counter() - ta
+
+    total_w = sum(nw for _, nw, _ in systems)
+    ctxs = contexts()
+    grp = m.AfqmcGroup(ctxs)
+    Tg = block(lambda t: grp.substep(t, t > 0 and t % northo == 0, 0.005), ctxs)
+    grp.close()
+    for g in ctxs:
+        g.close()
+    ctxs = contexts()
+    if args.graphs:
+        for g in ctxs:
+            g.use_graphs(True)
+
+    def seq(t):
+        for g in ctxs:
+            g.substep(t, t > 0 and t % northo == 0, 0.005)
+    Ts = block(seq, ctxs)
+    for g in ctxs:
+        g.close()
+    value = total_w * args.steps / Tg
+    vseq = total_w * args.steps / Ts
+    print(f"[bench] group: {Tg:.3f}s, one after another: {Ts:.3f}s for {args.steps} substeps", flush=True)
+    line = {
+        "metric": f"walker-steps/s ({len(systems)} uneven systems, M={systems[0][0].nmo}..{systems[-1][0].nmo}, "
+                  f"{total_w} walkers in total)",
+        "value": round(value, 2), "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * Tg / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": MODE_DTYPE[args.mode],
+        "data": "synthetic (seeded Hamiltonians and auxiliary fields)",
+        "config": {"workload": "multi", "systems": [[s.nmo, s.naea, s.nchol, nw] for s, nw, _ in systems],
+                   "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
+                   "energy": f"once per {spe} substeps", "parallelism": "single GPU, one graph launch per substep"},
+        "one_after_another": {"value": round(vseq, 2), "graphs": bool(args.graphs)},
+        "speedup_vs_one_after_another": round(value / vseq, 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +204,9 @@ def main():
 
     import miniafqmc_amd as m
     from miniafqmc_amd import _native as nat
+
+    if args.workload == "multi":
+        return bench_multi(args, m)
 
 
     t0 = time.time()
