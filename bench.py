@@ -93,6 +93,7 @@ def bench_multi(args, m):
         for s, nw, seed in systems:
             g = m.AfqmcSys(s, nw, args.mode, energy_precision=args.energy_level)
             g.set_exchange_chunk(args.exchange_chunk)
+            g.set_direct_lds(args.direct_lds)
             g.stage_fields(m.fields_block(seed, s.nchol, nw, 0, nsub))
             if args.tune:
                 g.tune(s.dt)
@@ -179,6 +180,9 @@ def main():
                          "of all timed substeps")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     ap.add_argument("--graphs", type=int, default=1, help="replay substeps from captured HIP graphs (0: eager)")
+    ap.add_argument("--direct-lds", type=int, default=1,
+                    help="split-BF16 GEMMs stage their split left operand by direct global-to-LDS loads (0: "
+                         "through registers)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
                          "hardware byte counters)")
@@ -251,6 +255,7 @@ def main():
         from `state`, or from this run's own final walkers (returned)."""
         g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level)
         g.set_exchange_chunk(args.exchange_chunk)
+        g.set_direct_lds(args.direct_lds)
         g.stage_fields(x)
         if args.tune:
             tt = time.perf_counter()
@@ -407,7 +412,7 @@ def main():
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
                    "exchange_chunk": args.exchange_chunk if args.energy_level < 3 else None,
-                   "graphs": bool(args.graphs),
+                   "graphs": bool(args.graphs), "direct_lds": bool(args.direct_lds),
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

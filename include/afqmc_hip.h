@@ -121,7 +121,13 @@ typedef enum {
    * kernel per walker and 32 columns, U kept in LDS; 0 (default) runs it as
    * eight GEMM launches, which measured 2 % faster at 512 orbitals.  Both
    * give the same walkers bit for bit.  May be changed at any time. */
-  AFQMC_OPT_FUSED_PROPAGATOR = 5
+  AFQMC_OPT_FUSED_PROPAGATOR = 5,
+  /* AFQMC_MIXED_BF16: 1 (default) stages the split-BF16 left operand of the
+   * GEMMs (Spvn, P, the walkers' vHS) from global memory straight into LDS
+   * with 16-byte direct loads where its layout allows; 0 passes it through
+   * registers.  Both give the same results bit for bit.  May be changed at
+   * any time.  No counterpart in the reference. */
+  AFQMC_OPT_DIRECT_LDS = 6
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

@@ -160,6 +160,11 @@ class AfqmcSys:
         """Split-BF16 mode: the whole propagator in one kernel (1) or as eight GEMMs (0)."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_FUSED_PROPAGATOR, 1.0 if on else 0.0))
 
+    def set_direct_lds(self, on):
+        """Split-BF16 mode: stage the GEMMs' split left operands by direct
+        global-to-LDS loads (1, default) or through registers (0)."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DIRECT_LDS, 1.0 if on else 0.0))
+
     def set_exchange_chunk(self, cols):
         """Dense exchange: accumulate the exchange GEMM over chunks of `cols`
         columns, adding the chunks' energies in FP64 (0: one pass)."""

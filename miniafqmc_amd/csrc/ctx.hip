@@ -128,6 +128,7 @@ struct afqmc_ctx {
   // orthogonalisation: Cholesky QR scratch (allocated at the first call)
   bool ortho_householder = false;  // always use the blocked Householder kernel
   bool fused_prop = false;         // split-BF16 propagator in one kernel when the shapes allow
+  bool gemm_adma = true;           // split-BF16 GEMMs stage A by direct global -> LDS loads
   long graph_gen = 0;              // bumped whenever captured substeps become stale
   long exch_chunk = 2048;          // dense exchange: columns per accumulation chunk (0: one pass)
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
@@ -316,6 +317,18 @@ int zgemm_cfg(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
   a.NT = (a.n + t.BNC - 1) / t.BNC;
   const long tiles = (long)a.MT * a.NT * batch;
   if (tiles > 2147483647L) return fail(c, AFQMC_ERR_SHAPE, "GEMM grid too large");
+  if constexpr (CP == CP_BF16X3 && std::is_same<AT, bfs>::value && AK) {
+    // direct global -> LDS staging of A: 16-byte loads of two complex at a
+    // time, so every row (and batch item) must start 16-byte aligned, and
+    // whole k-tiles (the triangular mode stops at column-block ends, which
+    // are multiples of 16 too)
+    if (c->gemm_adma && a.sak == 1 && a.k % BKC == 0 && a.sam % 2 == 0 && a.sa1 % 2 == 0 && a.sa2 % 2 == 0 &&
+        ((uintptr_t)a.A & 15) == 0 && (!a.ktri || t.BNC % BKC == 0))
+      return launch(c, name, [&]() {
+        hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN, true>),
+                           dim3((unsigned)tiles), dim3(64 * t.WM * t.WN), 0, c->stream, a);
+      });
+  }
   return launch(c, name, [&]() {
     hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN>), dim3((unsigned)tiles),
                        dim3(64 * t.WM * t.WN), 0, c->stream, a);
@@ -1554,6 +1567,11 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
   if (option == AFQMC_OPT_FUSED_PROPAGATOR) {
     drop_graphs(c);
     c->fused_prop = value != 0.0;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_DIRECT_LDS) {
+    drop_graphs(c);
+    c->gemm_adma = value != 0.0;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_ORTHO_HOUSEHOLDER) {

@@ -76,7 +76,12 @@ __device__ inline void zero_z(zb& z)
   z.lo = 0;
 }
 
-template <int CP, class AT, class BT, class OT, bool AK, bool BK, int BM, int BNC, int BKC, int WM, int WN>
+// ADMA (split-BF16 only): A is split-BF16 with unit k stride, 16-byte
+// aligned rows and k % 16 == 0 (checked by the launcher), and its k-tiles go
+// from global memory straight into LDS (global_load_lds_dwordx4), without
+// passing through registers.
+template <int CP, class AT, class BT, class OT, bool AK, bool BK, int BM, int BNC, int BKC, int WM, int WN,
+          bool ADMA = false>
 __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
 {
   constexpr int NT = 64 * WM * WN;
@@ -90,6 +95,8 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   static_assert(CP == CP_BF16X3 || BKC % 4 == 0, "k-tile must be a multiple of 4 complex");
   static_assert(CP == CP_BF16X3 || (!std::is_same<AT, bfs>::value && !std::is_same<BT, bfs>::value),
                 "split-BF16 operands need the BF16 path");
+  static_assert(!ADMA || (CP == CP_BF16X3 && std::is_same<AT, bfs>::value && AK && BKC == 16),
+                "direct LDS staging of A needs split-BF16 A with k contiguous and 16-complex k-tiles");
   using AZ = typename ztype<AT>::type;
   using BZ = typename ztype<BT>::type;
   using ACC = typename std::conditional<CP == CP_F64, doublex4, floatx4>::type;
@@ -248,21 +255,28 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   // of 4 consecutive k are both free of bank conflicts.
   constexpr int BKP = BKC / 2;
   constexpr int LDZ = BKC + 2;
-  constexpr int PA = (BM * BKP + NT - 1) / NT;
+  // With ADMA the A tile rows are 16 complex (128 B, eight 16-byte chunks,
+  // no padding, as the direct loads fill LDS contiguously) and chunk c of
+  // row R is stored at c ^ swz(R), swz(R) = (R & 7) ^ ((R >> 3) & 1): the 16
+  // rows that one fragment read touches then fall on 16 distinct 16-byte
+  // bank groups, and any 8 consecutive of them on 8.
+  constexpr int PA = ADMA ? 0 : (BM * BKP + NT - 1) / NT;  // A pairs per thread (register path)
+  constexpr int PAR = PA > 0 ? PA : 1;
   constexpr int PB = (BNC * BKP + NT - 1) / NT;
-  __shared__ __align__(16) uint2 zA[2][BM * LDZ];
+  constexpr int LDA = ADMA ? BKC : LDZ;
+  __shared__ __align__(16) uint2 zA[2][BM * LDA];
   __shared__ __align__(16) uint2 zB[2][BNC * LDZ];
   // global -> register prefetch of the next k-tile while the MFMAs run.
   // Each thread's operand pointers are fixed over the k loop; tiles that lie
   // inside the matrix in m / n load without per-element checks except in the
   // k-tile that crosses kend.
   static_assert((BM * BKP) % NT == 0 && (BNC * BKP) % NT == 0, "tile must split evenly over the threads");
-  AZ ra[PA][2];
+  AZ ra[PAR][2];
   BZ rb[PB][2];
-  const AZ* pa[PA];
+  const AZ* pa[PAR];
   const BZ* pb[PB];
-  int ka[PA], kb[PB];
-  bool va[PA], vb[PB];
+  int ka[PAR], kb[PB];
+  bool va[PAR], vb[PB];
 #pragma unroll
   for (int i = 0; i < PA; i++) {
     const int e = tid + i * NT;
@@ -283,6 +297,30 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
     kb[i] = 2 * kp;
     pb[i] = B + (long)(vb[i] ? gn : 0) * p.sbn + (long)kb[i] * p.sbk;
   }
+  // ADMA: 16-byte unit u = i * NT + tid of an A stage is (row u / 8, chunk
+  // u % 8), filled from logical chunk (u % 8) ^ swz(row); rows past m read
+  // row m - 1 (their outputs are never stored)
+  constexpr int AI = ADMA ? BM * BKP / NT : 1;  // direct loads per thread per k-tile
+  static_assert(!ADMA || (BM * BKP) % NT == 0, "A tile must split evenly over the threads");
+  const AZ* pd[AI];
+  if constexpr (ADMA) {
+#pragma unroll
+    for (int i = 0; i < AI; i++) {
+      const int u = i * NT + tid, row = u / BKP, pc = u % BKP;
+      const int lc = pc ^ ((row & 7) ^ ((row >> 3) & 1));
+      pd[i] = A + (long)min(m0 + row, p.m - 1) * p.sam + 2 * lc;
+    }
+  }
+  auto dma = [&](int st, int k0) {
+    if constexpr (ADMA) {
+      const int w0 = __builtin_amdgcn_readfirstlane(wave);
+#pragma unroll
+      for (int i = 0; i < AI; i++)
+        __builtin_amdgcn_global_load_lds((const void*)(pd[i] + k0),
+                                         (__attribute__((address_space(3))) void*)&zA[st][2 * (i * NT + w0 * 64)],
+                                         16, 0, 0);
+    }
+  };
   const bool interior = (m0 + BM <= p.m) && (n0 + BNC <= p.n);
   auto gload = [&](int k0) {
     const long oa = (long)k0 * p.sak, ob = (long)k0 * p.sbk;
@@ -333,7 +371,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int row, kp;
       if (AK) { row = e / BKP; kp = e % BKP; } else { kp = e / BM; row = e % BM; }
       const zb v0 = tozb(ra[i][0], false), v1 = tozb(ra[i][1], false);
-      *(uint4*)&zA[st][row * LDZ + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
+      *(uint4*)&zA[st][row * LDA + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
     }
 #pragma unroll
     for (int i = 0; i < PB; i++) {
@@ -349,14 +387,26 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   const int arow0 = wm * WTM + r;           // + bi*16
   const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
   const uint32_t brot = s ? 16u : 0u, bneg = s ? 0u : 0x80000000u;
-  auto compute = [&](int cur) {
+  // this lane's rows are 16-aligned blocks + r, so swz(row) = swz(r)
+  const int aswz = (r & 7) ^ ((r >> 3) & 1);
+  // knext >= 0 (ADMA): A's next k-tile is issued into the other stage once
+  // this tile's fragments are read
+  auto compute = [&](int cur, int knext) {
 #pragma unroll
     for (int kq = 0; kq < BKC / 16; kq++) {
       uint4 ah[TM], al[TM], bh[TN], bl[TN];
 #pragma unroll
       for (int i = 0; i < TM; i++) {
-        const uint4* pa = (const uint4*)&zA[cur][(arow0 + i * 16) * LDZ + kq * 16 + 4 * g];
-        const uint4 u0 = pa[0], u1 = pa[1];
+        uint4 u0, u1;
+        if constexpr (ADMA) {
+          const uint4* pa = (const uint4*)&zA[cur][(arow0 + i * 16) * LDA];
+          u0 = pa[(2 * g) ^ aswz];
+          u1 = pa[(2 * g + 1) ^ aswz];
+        } else {
+          const uint4* pa = (const uint4*)&zA[cur][(arow0 + i * 16) * LDA + kq * 16 + 4 * g];
+          u0 = pa[0];
+          u1 = pa[1];
+        }
         ah[i] = make_uint4(u0.x, u0.z, u1.x, u1.z);
         al[i] = make_uint4(u0.y, u0.w, u1.y, u1.w);
       }
@@ -371,6 +421,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         bh[j] = make_uint4(ex(u0.x), ex(u0.z), ex(u1.x), ex(u1.z));
         bl[j] = make_uint4(ex(u0.y), ex(u0.w), ex(u1.y), ex(u1.w));
       }
+      if (ADMA && kq == 0 && knext >= 0) dma(cur ^ 1, knext);
       // the three products of each accumulator are issued TM*TN MFMAs apart,
       // so consecutive MFMAs never wait on each other's result
 #pragma unroll
@@ -386,13 +437,14 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
     }
   };
   const int nk = (kend + BKC - 1) / BKC;
+  dma(0, 0);
   gload(0);
   sstore(0);
   __syncthreads();
   for (int kt = 0; kt < nk; kt++) {
     const int cur = kt & 1;
     if (kt + 1 < nk) gload((kt + 1) * BKC);
-    compute(cur);
+    compute(cur, kt + 1 < nk ? (kt + 1) * BKC : -1);
     // stage cur^1 was last read in iteration kt-1, before its barrier
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();

@@ -522,3 +522,36 @@ def test_fused_propagator_matches_gemm_chain(require_gpu, shape):
         out.append(g.W)
         g.close()
     assert np.array_equal(out[0], out[1]), f"max diff {np.abs(out[0] - out[1]).max():.3e}"
+
+
+@pytest.mark.parametrize("shape", [(40, 8, 64), (48, 12, 96), "N64_M256", "N128_M512"])
+def test_direct_lds_staging_matches_register_path(require_gpu, shape):
+    """Split-BF16 GEMMs whose left operand goes from global memory straight
+    into LDS (swizzled tile) give the walkers, overlaps and energies of the
+    register-staged path bit for bit; 40 and 48 orbitals leave partial row
+    tiles (rows past m read a clamped row whose outputs are dropped), and
+    at 40 orbitals the propagator's k = 40 is not a multiple of 16, so only
+    the bias and vHS products take the direct path."""
+    if isinstance(shape, str):
+        s, p = m.preset_system(shape)
+        seed = p["seed"]
+    else:
+        M, N, K = shape
+        s = m.make_system(M, N, K, density=0.5, sigma_L=0.02, seed=9)
+        seed = 9
+    nw = 16
+    x = m.fields_block(seed, s.nchol, nw, 0, 4)
+    out = []
+    for direct in (False, True):
+        say(f"direct LDS staging {shape} on={direct}")
+        g = m.AfqmcSys(s, nw, "mixed_bf16", dense_exchange=False)
+        g.set_direct_lds(direct)
+        g.init_walkers()
+        g.stage_fields(x)
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        g.run(1, 4, 2, s.dt)
+        out.append((g.W, g.W_data))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b), f"max diff {np.abs(a - b).max():.3e}"
