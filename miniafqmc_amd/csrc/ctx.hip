@@ -323,11 +323,20 @@ int zgemm_cfg(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
     // whole k-tiles (the triangular mode stops at column-block ends, which
     // are multiples of 16 too)
     if (c->gemm_adma && a.sak == 1 && a.k % BKC == 0 && a.sam % 2 == 0 && a.sa1 % 2 == 0 && a.sa2 % 2 == 0 &&
-        ((uintptr_t)a.A & 15) == 0 && (!a.ktri || t.BNC % BKC == 0))
+        ((uintptr_t)a.A & 15) == 0 && (!a.ktri || t.BNC % BKC == 0)) {
+      if constexpr (std::is_same<BT, bfs>::value && BK) {
+        // B too, on the same conditions
+        if (a.sbk == 1 && a.sbn % 2 == 0 && a.sb1 % 2 == 0 && a.sb2 % 2 == 0 && ((uintptr_t)a.B & 15) == 0)
+          return launch(c, name, [&]() {
+            hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN, true, true>),
+                               dim3((unsigned)tiles), dim3(64 * t.WM * t.WN), 0, c->stream, a);
+          });
+      }
       return launch(c, name, [&]() {
         hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN, true>),
                            dim3((unsigned)tiles), dim3(64 * t.WM * t.WN), 0, c->stream, a);
       });
+    }
   }
   return launch(c, name, [&]() {
     hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN>), dim3((unsigned)tiles),
@@ -794,6 +803,11 @@ int do_vhs(afqmc_ctx* c)
 // W <- P * T6(V) * P * W per walker and spin (afqmc_sys.hpp:146-173), T6 the
 // order-6 Taylor polynomial of exp(i V) applied as in apply_expM
 // (vHS.hpp:61-91): T_n = (i/n) V T_{n-1}, S += T_n.
+//
+// S and the Horner terms (Ta, Tb) hold each (walker, spin) M x N block
+// transposed, column j of the block contiguous ([N][M]): every GEMM of the
+// chain then reads them as a right operand with k contiguous, which the
+// split-BF16 path stages straight into LDS.
 template <int CP, class ST, class TT, class PT, class VT>
 int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
 {
@@ -817,12 +831,14 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     a.k = M;
     return a;
   };
-  // S = P * W
+  // S = P * W (S transposed)
   {
     ZGemmArgs a = base();
     a.A = P;
     a.B = c->W;
     a.C = c->S;
+    a.scm = 1;
+    a.scn = M;
     RET((zgemm<CP, PT, double, ST, true, false>(c, "prop_pw", a, 2 * nw)));
   }
   // exp(iV) S to sixth order (vHS.hpp:61-91) by Horner's rule:
@@ -836,13 +852,17 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     a.sa1 = MM;
     a.sa2 = 0;
     a.B = prev;
+    a.sbn = M;
+    a.sbk = 1;
     a.C = bufs[k & 1];
+    a.scm = 1;
+    a.scn = M;
     a.ar = 0.0;
     a.ai = 1.0 / (double)k;
     a.F = c->S;
     a.cf = 1;
-    if (k == 6) RET((zgemm<CP, VT, ST, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
-    else RET((zgemm<CP, VT, TT, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
+    if (k == 6) RET((zgemm<CP, VT, ST, TT, true, true>(c, "prop_taylor", a, 2 * nw)));
+    else RET((zgemm<CP, VT, TT, TT, true, true>(c, "prop_taylor", a, 2 * nw)));
     prev = bufs[k & 1];
   }
   // W = P * E6(V) S
@@ -850,8 +870,10 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     ZGemmArgs a = base();
     a.A = P;
     a.B = prev;
+    a.sbn = M;
+    a.sbk = 1;
     a.C = c->W;
-    RET((zgemm<CP, PT, TT, double, true, false>(c, "prop_ps", a, 2 * nw)));
+    RET((zgemm<CP, PT, TT, double, true, true>(c, "prop_ps", a, 2 * nw)));
   }
   return AFQMC_OK;
 }

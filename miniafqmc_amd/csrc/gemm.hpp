@@ -79,9 +79,9 @@ __device__ inline void zero_z(zb& z)
 // ADMA (split-BF16 only): A is split-BF16 with unit k stride, 16-byte
 // aligned rows and k % 16 == 0 (checked by the launcher), and its k-tiles go
 // from global memory straight into LDS (global_load_lds_dwordx4), without
-// passing through registers.
+// passing through registers.  BDMA (with ADMA): B likewise.
 template <int CP, class AT, class BT, class OT, bool AK, bool BK, int BM, int BNC, int BKC, int WM, int WN,
-          bool ADMA = false>
+          bool ADMA = false, bool BDMA = false>
 __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
 {
   constexpr int NT = 64 * WM * WN;
@@ -97,6 +97,9 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
                 "split-BF16 operands need the BF16 path");
   static_assert(!ADMA || (CP == CP_BF16X3 && std::is_same<AT, bfs>::value && AK && BKC == 16),
                 "direct LDS staging of A needs split-BF16 A with k contiguous and 16-complex k-tiles");
+  static_assert(!BDMA || (ADMA && std::is_same<BT, bfs>::value && BK),
+                "direct LDS staging of B needs split-BF16 B with k contiguous (and A staged likewise)");
+  static_assert(!BDMA || (BNC / WN) % 16 == 0, "each wave's B rows must start at a multiple of 16");
   using AZ = typename ztype<AT>::type;
   using BZ = typename ztype<BT>::type;
   using ACC = typename std::conditional<CP == CP_F64, doublex4, floatx4>::type;
@@ -259,24 +262,27 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
   // no padding, as the direct loads fill LDS contiguously) and chunk c of
   // row R is stored at c ^ swz(R), swz(R) = (R & 7) ^ ((R >> 3) & 1): the 16
   // rows that one fragment read touches then fall on 16 distinct 16-byte
-  // bank groups, and any 8 consecutive of them on 8.
+  // bank groups, and any 8 consecutive of them on 8.  With BDMA the B tile
+  // (rows = columns of the product) is laid out the same way.
   constexpr int PA = ADMA ? 0 : (BM * BKP + NT - 1) / NT;  // A pairs per thread (register path)
   constexpr int PAR = PA > 0 ? PA : 1;
-  constexpr int PB = (BNC * BKP + NT - 1) / NT;
+  constexpr int PB = BDMA ? 0 : (BNC * BKP + NT - 1) / NT;  // B pairs per thread (register path)
+  constexpr int PBR = PB > 0 ? PB : 1;
   constexpr int LDA = ADMA ? BKC : LDZ;
+  constexpr int LDB = BDMA ? BKC : LDZ;
   __shared__ __align__(16) uint2 zA[2][BM * LDA];
-  __shared__ __align__(16) uint2 zB[2][BNC * LDZ];
+  __shared__ __align__(16) uint2 zB[2][BNC * LDB];
   // global -> register prefetch of the next k-tile while the MFMAs run.
   // Each thread's operand pointers are fixed over the k loop; tiles that lie
   // inside the matrix in m / n load without per-element checks except in the
   // k-tile that crosses kend.
   static_assert((BM * BKP) % NT == 0 && (BNC * BKP) % NT == 0, "tile must split evenly over the threads");
   AZ ra[PAR][2];
-  BZ rb[PB][2];
+  BZ rb[PBR][2];
   const AZ* pa[PAR];
-  const BZ* pb[PB];
-  int ka[PAR], kb[PB];
-  bool va[PAR], vb[PB];
+  const BZ* pb[PBR];
+  int ka[PAR], kb[PBR];
+  bool va[PAR], vb[PBR];
 #pragma unroll
   for (int i = 0; i < PA; i++) {
     const int e = tid + i * NT;
@@ -311,6 +317,18 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       pd[i] = A + (long)min(m0 + row, p.m - 1) * p.sam + 2 * lc;
     }
   }
+  // BDMA: the same for B, rows past n reading row n - 1
+  constexpr int BI = BDMA ? BNC * BKP / NT : 1;
+  static_assert(!BDMA || (BNC * BKP) % NT == 0, "B tile must split evenly over the threads");
+  const BZ* pdb[BI];
+  if constexpr (BDMA) {
+#pragma unroll
+    for (int i = 0; i < BI; i++) {
+      const int u = i * NT + tid, row = u / BKP, pc = u % BKP;
+      const int lc = pc ^ ((row & 7) ^ ((row >> 3) & 1));
+      pdb[i] = B + (long)min(n0 + row, p.n - 1) * p.sbn + 2 * lc;
+    }
+  }
   auto dma = [&](int st, int k0) {
     if constexpr (ADMA) {
       const int w0 = __builtin_amdgcn_readfirstlane(wave);
@@ -319,6 +337,13 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         __builtin_amdgcn_global_load_lds((const void*)(pd[i] + k0),
                                          (__attribute__((address_space(3))) void*)&zA[st][2 * (i * NT + w0 * 64)],
                                          16, 0, 0);
+      if constexpr (BDMA) {
+#pragma unroll
+        for (int i = 0; i < BI; i++)
+          __builtin_amdgcn_global_load_lds((const void*)(pdb[i] + k0),
+                                           (__attribute__((address_space(3))) void*)&zB[st][2 * (i * NT + w0 * 64)],
+                                           16, 0, 0);
+      }
     }
   };
   const bool interior = (m0 + BM <= p.m) && (n0 + BNC <= p.n);
@@ -380,13 +405,17 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       int col, kp;
       if (BK) { col = e / BKP; kp = e % BKP; } else { kp = e / BNC; col = e % BNC; }
       const zb v0 = tozb(rb[i][0], p.conjb != 0), v1 = tozb(rb[i][1], p.conjb != 0);
-      *(uint4*)&zB[st][col * LDZ + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
+      *(uint4*)&zB[st][col * LDB + 2 * kp] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
     }
   };
 
   const int arow0 = wm * WTM + r;           // + bi*16
   const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
-  const uint32_t brot = s ? 16u : 0u, bneg = s ? 0u : 0x80000000u;
+  const uint32_t brot = s ? 16u : 0u;
+  // BDMA: conj(B) is taken here (the register path takes it at the LDS
+  // store): the sign bits of the imaginary halves, bit 31 of each word,
+  // move to bit 15 in the rotated words of the s = 1 lanes
+  const uint32_t bneg = (s ? 0u : 0x80000000u) ^ ((BDMA && p.conjb) ? (s ? 0x00008000u : 0x80000000u) : 0u);
   // this lane's rows are 16-aligned blocks + r, so swz(row) = swz(r)
   const int aswz = (r & 7) ^ ((r >> 3) & 1);
   // knext >= 0 (ADMA): A's next k-tile is issued into the other stage once
@@ -412,8 +441,19 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
       }
 #pragma unroll
       for (int j = 0; j < TN; j++) {
-        const uint4* pb = (const uint4*)&zB[cur][(bcol0 + j * 8) * LDZ + kq * 16 + 4 * g];
-        const uint4 u0 = pb[0], u1 = pb[1];
+        uint4 u0, u1;
+        if constexpr (BDMA) {
+          // column-block rows bcol0 + 8j: (wn * WTN) / 2 is a multiple of 16,
+          // so swz(row) = (r / 2) ^ (j & 1)
+          const uint4* pb = (const uint4*)&zB[cur][(bcol0 + j * 8) * LDB];
+          const int bsw = (r >> 1) ^ (j & 1);
+          u0 = pb[(2 * g) ^ bsw];
+          u1 = pb[(2 * g + 1) ^ bsw];
+        } else {
+          const uint4* pb = (const uint4*)&zB[cur][(bcol0 + j * 8) * LDB + kq * 16 + 4 * g];
+          u0 = pb[0];
+          u1 = pb[1];
+        }
         // lanes s = 0 (real output column) take (b_re, -b_im), lanes s = 1
         // (imaginary column) take (b_im, b_re): one xor and one rotate per
         // word, no branch
