@@ -378,25 +378,28 @@ int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
   if (c->force_cfg > 0 && c->force_cfg <= ncfg<CP>())
     return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, c->force_cfg - 1);
   if (!c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, default_cfg<CP>(a));
-  // tuning: time each candidate (results of this call are discarded)
+  // tuning: time each candidate (results of this call are discarded), in
+  // two rounds keeping each candidate's faster round
   const bool prof = c->prof;
   c->prof = false;
   hipEvent_t e0 = get_event(c), e1 = get_event(c);
-  int best = default_cfg<CP>(a);
-  float best_ms = 1e30f;
-  for (int ci = 0; ci < ncfg<CP>(); ci++) {
-    RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));  // warm
-    (void)hipEventRecord(e0, c->stream);
-    for (int rep = 0; rep < 3; rep++) RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));
-    (void)hipEventRecord(e1, c->stream);
-    (void)hipEventSynchronize(e1);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = ci;
+  const int def = default_cfg<CP>(a);
+  float tms[8];
+  for (int ci = 0; ci < ncfg<CP>(); ci++) tms[ci] = 1e30f;
+  for (int round = 0; round < 2; round++)
+    for (int ci = 0; ci < ncfg<CP>(); ci++) {
+      RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));  // warm
+      (void)hipEventRecord(e0, c->stream);
+      for (int rep = 0; rep < 3; rep++) RET((zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, ci)));
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      tms[ci] = std::min(tms[ci], ms);
     }
-  }
+  int best = def;
+  for (int ci = 0; ci < ncfg<CP>(); ci++)
+    if (tms[ci] < tms[best]) best = ci;
   c->evpool.push_back(e0);
   c->evpool.push_back(e1);
   c->prof = prof;
@@ -804,10 +807,11 @@ int do_vhs(afqmc_ctx* c)
 // order-6 Taylor polynomial of exp(i V) applied as in apply_expM
 // (vHS.hpp:61-91): T_n = (i/n) V T_{n-1}, S += T_n.
 //
-// S and the Horner terms (Ta, Tb) hold each (walker, spin) M x N block
-// transposed, column j of the block contiguous ([N][M]): every GEMM of the
-// chain then reads them as a right operand with k contiguous, which the
-// split-BF16 path stages straight into LDS.
+// The Horner terms (Ta, Tb) hold each (walker, spin) M x N block
+// transposed, column j of the block contiguous ([N][M]): the Taylor steps
+// and P U_1 then read them as a right operand with k contiguous, which the
+// split-BF16 path stages straight into LDS.  S stays row-major (it is read
+// as the first step's right operand, in FP32, and as the epilogues' F).
 template <int CP, class ST, class TT, class PT, class VT>
 int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
 {
@@ -831,14 +835,12 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     a.k = M;
     return a;
   };
-  // S = P * W (S transposed)
+  // S = P * W
   {
     ZGemmArgs a = base();
     a.A = P;
     a.B = c->W;
     a.C = c->S;
-    a.scm = 1;
-    a.scn = M;
     RET((zgemm<CP, PT, double, ST, true, false>(c, "prop_pw", a, 2 * nw)));
   }
   // exp(iV) S to sixth order (vHS.hpp:61-91) by Horner's rule:
@@ -852,16 +854,20 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
     a.sa1 = MM;
     a.sa2 = 0;
     a.B = prev;
-    a.sbn = M;
-    a.sbk = 1;
+    if (k < 6) {  // U_{k+1} transposed; U_7 = S row-major
+      a.sbn = M;
+      a.sbk = 1;
+    }
     a.C = bufs[k & 1];
     a.scm = 1;
     a.scn = M;
     a.ar = 0.0;
     a.ai = 1.0 / (double)k;
     a.F = c->S;
+    a.sfm = N;
+    a.sfn = 1;
     a.cf = 1;
-    if (k == 6) RET((zgemm<CP, VT, ST, TT, true, true>(c, "prop_taylor", a, 2 * nw)));
+    if (k == 6) RET((zgemm<CP, VT, ST, TT, true, false>(c, "prop_taylor", a, 2 * nw)));
     else RET((zgemm<CP, VT, TT, TT, true, true>(c, "prop_taylor", a, 2 * nw)));
     prev = bufs[k & 1];
   }

@@ -53,6 +53,9 @@ struct ZGemmArgs {
   int conjb;
   // C(m,n) = F(m,n) + alpha*AB (F with C's strides, D unused)
   int cf;
+  // F(m,n) at F[m*sfm + n*sfn] (complex elements; batch offsets as C's)
+  // when sfm or sfn is nonzero, else with C's strides
+  long sfm, sfn;
 };
 
 template <int CP> struct LdsT;
@@ -514,7 +517,8 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
         const int row = rbase + i * 16 + RQ * q, col = cbase + j * 8;
         const bool in = row < p.m && col < p.n;
         const long off = 2 * ((long)row * p.scm + (long)col * p.scn);
-        fv[j][q] = ((D || p.cf) && F && in) ? F[off + s] : (DT)0;
+        const long foff = (p.sfm || p.sfn) ? 2 * ((long)row * p.sfm + (long)col * p.sfn) : off;
+        fv[j][q] = ((D || p.cf) && F && in) ? F[foff + s] : (DT)0;
         if constexpr (!OSPLIT) cv[j][q] = (C && has_beta && in) ? C[off + s] : (CT)0;
       }
 #pragma unroll
