@@ -40,8 +40,8 @@ MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128"
 # counters, scaled by a 1 GiB copy), per (call site, workload, mode);
 # profiles/walker_step_N128_M512_hbm_traffic.md has the commands and figures.
 MEASURED_TRAFFIC = {
-    # Horner step: 4.01 GiB read (V, U, S) + 1.00 GiB written (U') per launch
-    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.382e9,
+    # Horner step: 4.03 GiB read (V, U, S) + 1.00 GiB written (U') per launch
+    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.403e9,
 }
 
 

@@ -800,7 +800,10 @@ int do_vhs(afqmc_ctx* c)
   a.k = K;
   if (c->mode == AFQMC_FP64_DENSE) return zgemm<CP_F64, double, double, double, true, false>(c, "vhs_gemm", a, 1);
   if (c->mode == AFQMC_MIXED_FP32) return zgemm<CP_F32, float, float, float, true, false>(c, "vhs_gemm", a, 1);
-  return zgemm<CP_BF16X3, bfs, bfs, bfs, true, false>(c, "vhs_gemm", a, 1);
+  // split-BF16: Ld stored [M*M][K]
+  a.sbn = K;
+  a.sbk = 1;
+  return zgemm<CP_BF16X3, bfs, bfs, bfs, true, true>(c, "vhs_gemm", a, 1);
 }
 
 // W <- P * T6(V) * P * W per walker and spin (afqmc_sys.hpp:146-173), T6 the
@@ -1798,8 +1801,10 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
       RET(dalloc(c, &Lt32, 2 * K * NM));
       RET(launch(c, "convert", [&]() {
         if (c->bsplit) {
-          hipLaunchKernelGGL((zconvert_kernel<zd, zb>), dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, (zb*)Ld32,
-                             K * MM);
+          // split-BF16 Ld is stored transposed, [M*M][K]: the vHS GEMM reads
+          // it with k contiguous and stages it straight into LDS
+          hipLaunchKernelGGL((ztranspose_kernel<zd, zb>), dim3((unsigned)((MM + 31) / 32), (unsigned)((K + 31) / 32)),
+                             dim3(256), 0, st, Ld64, (zb*)Ld32, (int)K, (int)MM, MM, K);
           hipLaunchKernelGGL((zconvert_kernel<zd, zb>), dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64,
                              (zb*)Lt32, 2 * K * NM);
         } else {
