@@ -325,8 +325,9 @@ int zgemm_cfg(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
     if (c->gemm_adma && a.sak == 1 && a.k % BKC == 0 && a.sam % 2 == 0 && a.sa1 % 2 == 0 && a.sa2 % 2 == 0 &&
         ((uintptr_t)a.A & 15) == 0 && (!a.ktri || t.BNC % BKC == 0)) {
       if constexpr (std::is_same<BT, bfs>::value && BK) {
-        // B too, on the same conditions
-        if (a.sbk == 1 && a.sbn % 2 == 0 && a.sb1 % 2 == 0 && a.sb2 % 2 == 0 && ((uintptr_t)a.B & 15) == 0)
+        // B too, on the same conditions (conj(B) only through registers)
+        if (a.sbk == 1 && a.sbn % 2 == 0 && a.sb1 % 2 == 0 && a.sb2 % 2 == 0 && ((uintptr_t)a.B & 15) == 0 &&
+            !a.conjb)
           return launch(c, name, [&]() {
             hipLaunchKernelGGL((zgemm_kernel<CP, AT, BT, OT, AK, BK, t.BM, t.BNC, BKC, t.WM, t.WN, true, true>),
                                dim3((unsigned)tiles), dim3(64 * t.WM * t.WN), 0, c->stream, a);

@@ -414,11 +414,8 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
 
   const int arow0 = wm * WTM + r;           // + bi*16
   const int bcol0 = (wn * WTN) / 2 + r / 2; // + bj*8  (complex column in tile)
-  const uint32_t brot = s ? 16u : 0u;
-  // BDMA: conj(B) is taken here (the register path takes it at the LDS
-  // store): the sign bits of the imaginary halves, bit 31 of each word,
-  // move to bit 15 in the rotated words of the s = 1 lanes
-  const uint32_t bneg = (s ? 0u : 0x80000000u) ^ ((BDMA && p.conjb) ? (s ? 0x00008000u : 0x80000000u) : 0u);
+  // (conj(B) is taken at the LDS store, so BDMA launches have conjb == 0)
+  const uint32_t brot = s ? 16u : 0u, bneg = s ? 0u : 0x80000000u;
   // this lane's rows are 16-aligned blocks + r, so swz(row) = swz(r)
   const int aswz = (r & 7) ^ ((r >> 3) & 1);
   // knext >= 0 (ADMA): A's next k-tile is issued into the other stage once
