@@ -70,3 +70,21 @@ def test_product_imports_no_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")) or f == "Makefile":
                 txt = open(os.path.join(dp, f), errors="ignore").read()
                 assert "oracle" not in txt.lower(), f
+
+
+def test_python_constants_match_header():
+    """Every status code, mode, buffer id and option of include/afqmc_hip.h
+    has the same value in the Python mirror, and the mirror has no extra ones."""
+    import miniafqmc_amd._native as nat
+    text = open(os.path.join(ROOT, "include", "afqmc_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    enum = {k: int(v) for k, v in re.findall(r"\b(AFQMC_[A-Z0-9_]+)\s*=\s*(-?\d+)", text)}
+    status = {k: v for k, v in enum.items() if k == "AFQMC_OK" or k.startswith("AFQMC_ERR_")}
+    assert {v: k for k, v in status.items()} == nat.STATUS
+    modes = {k[len("AFQMC_"):].lower(): v for k, v in enum.items()
+             if k in ("AFQMC_FP64_CSR", "AFQMC_FP64_DENSE", "AFQMC_MIXED_FP32", "AFQMC_MIXED_BF16")}
+    assert modes == nat.MODES
+    for prefix in ("AFQMC_BUF_", "AFQMC_OPT_"):
+        hdr = {k[len("AFQMC_"):]: v for k, v in enum.items() if k.startswith(prefix)}
+        py = {k: getattr(nat, k) for k in dir(nat) if k.startswith(prefix[len("AFQMC_"):])}
+        assert hdr == py, (prefix, hdr, py)
