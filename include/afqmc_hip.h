@@ -223,6 +223,18 @@ int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int k, const do
 /* Device copy of the real N(0,1) draws of `nsub` substeps, x_all
  * [nsub][K][nwalk]; afqmc_fields(ctx, NULL, s) and afqmc_run use them. */
 int afqmc_stage_fields(afqmc_ctx* ctx, const double* x_all, long nsub);
+/* Draw the auxiliary fields on the device, inside each substep, as the
+ * reference draws them inside its substep loop (Random.generate_normal,
+ * src/miniapps/miniafqmc_base.cpp:282, Box-Muller of
+ * src/Utilities/RandomGenerator.h:50-62).  enable != 0: afqmc_fields(ctx,
+ * NULL, s), afqmc_substep, afqmc_run and afqmc_tune generate the draws of
+ * substep s for walker w from (seed, s, w), bit for bit the numbers
+ * afqmc_synth_fields(seed, K, nwalk, 0, nwalk, s, x) returns on the host (so
+ * the same run can be replayed on the CPU), and need no staged block.
+ * enable == 0: the staged fields are used again.  The stream itself differs
+ * from the reference's Mersenne-Twister stream (a counter-based generator, so
+ * a captured substep can draw in parallel). */
+int afqmc_device_fields(afqmc_ctx* ctx, uint64_t seed, int enable);
 /* One substep (miniafqmc_base.cpp:251-338): every phase above, with the
  * orthogonalisation and overlap refresh when do_ortho != 0.  Draws come from
  * the staged fields of `substep`.  Eshift is kept on the device; read it with

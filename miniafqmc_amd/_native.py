@@ -36,7 +36,7 @@ EXPORTED = [
     "afqmc_fields", "afqmc_vhs",
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
     "afqmc_weights", "afqmc_branch", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
-    "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
+    "afqmc_device_fields", "afqmc_substep", "afqmc_get_eshift", "afqmc_set_eshift", "afqmc_run", "afqmc_tune",
     "afqmc_use_graphs", "afqmc_profile", "afqmc_kernel_stats", "afqmc_zgemm",
     "afqmc_group_create", "afqmc_group_destroy", "afqmc_group_substep", "afqmc_group_sync", "afqmc_group_error",
     "afqmc_synth_create", "afqmc_synth_destroy", "afqmc_synth_get", "afqmc_synth_fields",
@@ -116,6 +116,7 @@ def _load():
         "afqmc_orthogonalize": (I, [P]),
         "afqmc_energy": (I, [P, D]),
         "afqmc_stage_fields": (I, [P, D, L]),
+        "afqmc_device_fields": (I, [P, C.c_uint64, I]),
         "afqmc_substep": (I, [P, L, I, C.c_double]),
         "afqmc_get_eshift": (I, [P, D]),
         "afqmc_set_eshift": (I, [P, C.c_double]),

@@ -202,6 +202,12 @@ class AfqmcSys:
         assert xa.ndim == 3 and xa.shape[1:] == (self.K, self.nwalk)
         self._check(nat.lib.afqmc_stage_fields(self.ctx, _dp(xa), xa.shape[0]))
 
+    def device_fields(self, seed, enable=True):
+        """Draw the fields on the device inside each substep (the reference's
+        generate_normal at miniafqmc_base.cpp:282), bit for bit the draws of
+        ``system.fields(seed, K, nwalk, substep)``; no staged block needed."""
+        self._check(nat.lib.afqmc_device_fields(self.ctx, int(seed), 1 if enable else 0))
+
     def substep(self, s, do_ortho, dt):
         self._check(nat.lib.afqmc_substep(self.ctx, s, int(do_ortho), dt))
 

@@ -110,6 +110,10 @@ struct afqmc_ctx {
   int* flags = nullptr;   // [0] singular count, [1] non-finite fields
   double* xstage = nullptr;
   long nstage = 0;
+  // fields drawn on the device from (field_seed, substep, walker) instead of
+  // a staged block (afqmc_device_fields)
+  bool dev_fields = false;
+  uint64_t field_seed = 0;
   // energy scratch
   zd* e_direct = nullptr;
   zd* e_exch = nullptr;
@@ -745,30 +749,40 @@ int do_vbias(afqmc_ctx* c)
   return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
 }
 
-// xslice != null: x is the staged block and *xslice the substep index
-int do_fields(afqmc_ctx* c, const double* xdev, const long* xslice = nullptr)
+// Draws of the fields phase: xdev (a device block [K][nw]; with xslice !=
+// null the staged block and *xslice its substep index) or, when xdev is null,
+// generated on the device from (field_seed, substep or *xslice, walker).
+int do_fields(afqmc_ctx* c, const double* xdev, long substep, const long* xslice = nullptr)
 {
   const int nw = c->nw, K = c->K;
-  dim3 grid((nw + 63) / 64);
+  FieldSrc src{xdev, xslice, substep, c->field_seed};
   if (c->mode == AFQMC_FP64_CSR)
     return launch(c, "fields", [&]() {
-      hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, (long)nw, 1L, xdev,
-                         (zd*)c->X, (long)nw, 1L, c->hybridW, c->flags + 1, xslice);
-    });
-  if (!c->mixed)
-    return launch(c, "fields", [&]() {
-      hipLaunchKernelGGL((fields_kernel<zd, zd>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                         (zd*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
-    });
-  if (c->bsplit)
-    return launch(c, "fields", [&]() {
-      hipLaunchKernelGGL((fields_kernel<zd, zb>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                         (zb*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
+      hipLaunchKernelGGL((fields_wlast_kernel<zd>), dim3((nw + 63) / 64), dim3(1024), 0, c->stream, K, nw, c->vbias,
+                         src, (zd*)c->X, c->hybridW, c->flags + 1);
     });
   return launch(c, "fields", [&]() {
-    hipLaunchKernelGGL((fields_kernel<zd, zf>), grid, dim3(256), 0, c->stream, K, nw, c->vbias, 1L, (long)K, xdev,
-                       (zf*)c->X, 1L, (long)K, c->hybridW, c->flags + 1, xslice);
+    if (!c->mixed)
+      hipLaunchKernelGGL((fields_wfirst_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
+                         (zd*)c->X, c->hybridW, c->flags + 1);
+    else if (c->bsplit)
+      hipLaunchKernelGGL((fields_wfirst_kernel<zb>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
+                         (zb*)c->X, c->hybridW, c->flags + 1);
+    else
+      hipLaunchKernelGGL((fields_wfirst_kernel<zf>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
+                         (zf*)c->X, c->hybridW, c->flags + 1);
   });
+}
+
+// The fields of substep s are available: drawn on the device, or staged.
+bool have_fields(const afqmc_ctx* c, long s)
+{
+  return s >= 0 && (c->dev_fields || (c->xstage && s < c->nstage));
+}
+// the staged draws of substep s, or null when they are drawn on the device
+const double* stage_at(const afqmc_ctx* c, long s)
+{
+  return c->dev_fields ? nullptr : c->xstage + s * (long)c->K * c->nw;
 }
 
 int do_vhs(afqmc_ctx* c)
@@ -1347,7 +1361,8 @@ int check_flags(afqmc_ctx* c)
   return AFQMC_OK;
 }
 
-int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt, const long* xslice = nullptr)
+int substep_body(afqmc_ctx* c, const double* xdev, long substep, int do_ortho_, double dt,
+                 const long* xslice = nullptr)
 {
   if (c->transposed) {
     RET(do_mdm(c));
@@ -1356,7 +1371,7 @@ int substep_body(afqmc_ctx* c, const double* xdev, int do_ortho_, double dt, con
     RET(do_mdm_full(c));
     RET(do_vbias_full(c));
   }
-  RET(do_fields(c, xdev, xslice));
+  RET(do_fields(c, xdev, substep, xslice));
   RET(do_vhs(c));
   RET(do_propagate(c));
   RET(do_save_overlaps(c));
@@ -1430,7 +1445,7 @@ int capture_substep(afqmc_ctx* c, int ortho, double dt)
 {
   RET(capture_prep(c, ortho));
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
+  const int r = substep_body(c, c->dev_fields ? nullptr : c->xstage, 0, ortho, dt, c->d_sub);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (r != AFQMC_OK) {
@@ -2003,10 +2018,10 @@ int afqmc_fields(afqmc_ctx* c, const double* x, long substep)
     HIPCHK(c, hipMemcpyAsync(tmp, x, sizeof(double) * KW, hipMemcpyHostToDevice, c->stream));
     xdev = tmp;
   } else {
-    if (!c->xstage || substep < 0 || substep >= c->nstage) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
-    xdev = c->xstage + substep * KW;
+    if (!have_fields(c, substep)) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
+    xdev = stage_at(c, substep);
   }
-  int r = do_fields(c, xdev);
+  int r = do_fields(c, xdev, substep);
   if (r == AFQMC_OK) r = check_flags(c);
   if (tmp) {
     (void)hipStreamSynchronize(c->stream);
@@ -2080,13 +2095,21 @@ int afqmc_stage_fields(afqmc_ctx* c, const double* x_all, long nsub)
   return AFQMC_OK;
 }
 
+int afqmc_device_fields(afqmc_ctx* c, uint64_t seed, int enable)
+{
+  if (!c) return AFQMC_ERR_ARG;
+  drop_graphs(c);  // captured substeps read the fields from the old source
+  c->dev_fields = enable != 0;
+  c->field_seed = seed;
+  return AFQMC_OK;
+}
+
 int afqmc_substep(afqmc_ctx* c, long substep, int do_ortho_, double dt)
 {
   NEED_SYSTEM(c);
-  if (!c->xstage || substep < 0 || substep >= c->nstage) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
-  const long KW = (long)c->K * c->nw;
+  if (!have_fields(c, substep)) return fail(c, AFQMC_ERR_STATE, "no staged fields for substep");
   const int o = do_ortho_ ? 1 : 0;
-  if (!c->use_graphs || c->prof || c->tuning) return substep_body(c, c->xstage + substep * KW, do_ortho_, dt);
+  if (!c->use_graphs || c->prof || c->tuning) return substep_body(c, stage_at(c, substep), substep, do_ortho_, dt);
   if (c->g_dt != dt) {
     drop_graphs(c);
     c->g_dt = dt;
@@ -2122,7 +2145,7 @@ int afqmc_run(afqmc_ctx* c, long first, int nsteps, int nsubsteps, int northo, d
 {
   NEED_SYSTEM(c);
   if (nsteps < 0 || nsubsteps < 0 || northo <= 0) return AFQMC_ERR_ARG;
-  if (!c->xstage || first < 0 || first + (long)nsteps * nsubsteps > c->nstage)
+  if (first < 0 || (!c->dev_fields && (!c->xstage || first + (long)nsteps * nsubsteps > c->nstage)))
     return fail(c, AFQMC_ERR_STATE, "not enough staged fields");
   long tot = first;
   for (int step = 0; step < nsteps; step++) {
@@ -2141,13 +2164,13 @@ int afqmc_run(afqmc_ctx* c, long first, int nsteps, int nsubsteps, int northo, d
 int afqmc_tune(afqmc_ctx* c, double dt)
 {
   NEED_SYSTEM(c);
-  if (!c->xstage || c->nstage < 1) return fail(c, AFQMC_ERR_STATE, "stage fields before tuning");
+  if (!have_fields(c, 0)) return fail(c, AFQMC_ERR_STATE, "stage fields before tuning");
   // tune on walkers at the trial (a representative, well-conditioned state)
   drop_graphs(c);
   int r = afqmc_init_walkers(c);
   if (r != AFQMC_OK) return r;
   c->tuning = true;
-  r = substep_body(c, c->xstage, 1, dt);
+  r = substep_body(c, stage_at(c, 0), 0, 1, dt);
   if (r == AFQMC_OK) r = do_mdm(c);
   if (r == AFQMC_OK) r = do_energy(c, nullptr);
   c->tuning = false;
@@ -2331,7 +2354,7 @@ int group_capture(afqmc_group* g, int ortho, double dt)
       r = AFQMC_ERR_HIP;
       break;
     }
-    r = substep_body(c, c->xstage, ortho, dt, c->d_sub);
+    r = substep_body(c, c->dev_fields ? nullptr : c->xstage, 0, ortho, dt, c->d_sub);
     if (r != AFQMC_OK) g->err = std::string("member: ") + c->err;
     if (r == AFQMC_OK && (hipEventRecord(g->join[i], c->stream) != hipSuccess ||
                           hipStreamWaitEvent(g->stream, g->join[i], 0) != hipSuccess))
@@ -2401,7 +2424,7 @@ int afqmc_group_substep(afqmc_group* g, long substep, int do_ortho, double dt)
   if (!g) return AFQMC_ERR_ARG;
   const int o = do_ortho ? 1 : 0;
   for (afqmc_ctx* c : g->ctx)
-    if (!c->xstage || substep < 0 || substep >= c->nstage)
+    if (!have_fields(c, substep))
       return gfail(g, AFQMC_ERR_STATE, "a member has no staged fields for this substep");
   // profiling and tuning run the members' substeps eagerly, one by one
   bool eager = false;

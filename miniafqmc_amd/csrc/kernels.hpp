@@ -1,6 +1,7 @@
 // Non-GEMM kernels of the walker step (FP64 unless stated).
 #pragma once
 #include "common.hpp"
+#include "fieldrng.hpp"
 
 // ---------------------------------------------------------------------------
 // Batched in-place Gauss-Jordan inverse + determinant of N x N complex
@@ -438,38 +439,108 @@ __global__ void zaxpy1_kernel(const zd* __restrict__ x, zd* __restrict__ y, long
 }
 
 // ---------------------------------------------------------------------------
-// X / hybrid weight (src/miniapps/miniafqmc_base.cpp:281-289).  Element (n, w)
-// of vbias at vb[n*vsn + w*vsw], of x at x[n*nw + w] (draws, [K][nw]), of X at
-// X[n*xsn + w*xsw].  hybridW[w] = -sum_n i*vb*(x + i/2*vb), X = x + i*vb.
-// One block per 64 walkers x (n split over 4 waves), deterministic reduction.
-template <class VT, class XT>
-__global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __restrict__ vb, long vsn, long vsw,
-                                                     const double* __restrict__ x, XT* __restrict__ X, long xsn,
-                                                     long xsw, zd* __restrict__ hybridW, int* __restrict__ nonfinite,
-                                                     const long* __restrict__ xslice)
+// X / hybrid weight (src/miniapps/miniafqmc_base.cpp:281-289):
+//   X(n, w) = x(n, w) + i vbias(n, w),  hybridW[w] = -sum_n i vb (x + i/2 vb).
+// The real draws x come from a staged block (x(n, w) at x[n*nw + w]; with
+// xslice != null the block of substep *xslice, an index kept in device memory
+// so that a captured graph can be replayed) or, when x is null, are generated
+// on the fly from (seed, substep, walker) exactly as afqmc_synth_fields does
+// on the host (the reference draws them inside the substep,
+// miniafqmc_base.cpp:282).  Non-finite staged draws raise *nonfinite.
+struct FieldSrc {
+  const double* x;      // staged draws (null: generate)
+  const long* xslice;   // device substep index, or null
+  long substep;         // substep index when xslice is null
+  uint64_t seed;        // generator seed (x == null)
+};
+
+__device__ inline void field_term(zd b, double xr, zd& h, zd& Xo)
 {
-  // xslice != null: the draws of substep *xslice of the staged block (the
-  // index lives in device memory so a captured graph can be replayed)
-  if (xslice) x += *xslice * (long)K * nw;
-  __shared__ zd part[4][64];
+  // t1 = i*vb ; t3 = x + i/2*vb ; hybridW -= t1*t3 ; X = x + i*vb
+  const zd t1{-b.im, b.re};
+  const zd t3{xr - 0.5 * b.im, 0.5 * b.re};
+  const zd t4 = zmul(t1, t3);
+  h.re -= t4.re;
+  h.im -= t4.im;
+  Xo = zd{xr - b.im, b.re};
+}
+
+// Walker-first layouts (vbias, X [nw][K]): one workgroup per walker, the
+// lanes along n (coalesced), each thread a pair (2j, 2j+1) of draws at a
+// time; fixed-order block reduction of hybridW.
+template <class XT>
+__global__ __launch_bounds__(256) void fields_wfirst_kernel(int K, int nw, const zd* __restrict__ vb, FieldSrc src,
+                                                            XT* __restrict__ X, zd* __restrict__ hybridW,
+                                                            int* __restrict__ nonfinite)
+{
+  __shared__ zd part[4];
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long sub = src.xslice ? *src.xslice : src.substep;
+  const double* x = src.x ? src.x + (src.xslice ? sub * (long)K * nw : 0L) : nullptr;
+  const zd* vbw = vb + (long)w * K;
+  XT* Xw = X + (long)w * K;
+  zd h{0, 0};
+  bool bad = false;
+  for (int n = 2 * threadIdx.x; n < K; n += 512) {
+    double xa, xb = 0.0;
+    if (x) {
+      xa = x[(long)n * nw + w];
+      if (n + 1 < K) xb = x[(long)(n + 1) * nw + w];
+      bad |= !isfinite(xa) || !isfinite(xb);
+    } else {
+      afqmc_rng::field_pair(src.seed, sub, w, n, xa, xb);
+    }
+    zd o;
+    field_term(vbw[n], xa, h, o);
+    from_zd(o, Xw[n]);
+    if (n + 1 < K) {
+      field_term(vbw[n + 1], xb, h, o);
+      from_zd(o, Xw[n + 1]);
+    }
+  }
+  if (bad) atomicOr(nonfinite, 1);
+  for (int off = 32; off >= 1; off >>= 1) {
+    h.re += __shfl_xor(h.re, off);
+    h.im += __shfl_xor(h.im, off);
+  }
+  if (lane == 0) part[wv] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) hybridW[w] = zadd(zadd(part[0], part[1]), zadd(part[2], part[3]));
+}
+
+// Walker-last layout (vbias, X [K][nw], the FP64 CSR mode): one workgroup per
+// 64 walkers (lanes), its 16 waves taking pairs (2j, 2j+1) of n in turn;
+// fixed-order reduction over the waves.
+template <class XT>
+__global__ __launch_bounds__(1024) void fields_wlast_kernel(int K, int nw, const zd* __restrict__ vb, FieldSrc src,
+                                                            XT* __restrict__ X, zd* __restrict__ hybridW,
+                                                            int* __restrict__ nonfinite)
+{
+  __shared__ zd part[16][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;
+  const long sub = src.xslice ? *src.xslice : src.substep;
+  const double* x = src.x ? src.x + (src.xslice ? sub * (long)K * nw : 0L) : nullptr;
   zd h{0, 0};
   bool bad = false;
   if (w < nw) {
-    for (int n = wv; n < K; n += 4) {
-      const zd b = to_zd(vb[(long)n * vsn + (long)w * vsw]);
-      double xr = x[(long)n * nw + w];
-      if (!isfinite(xr)) bad = true;
-      // t1 = i*vb ; t3 = x + i/2*vb ; hybridW -= t1*t3
-      zd t1{-b.im, b.re};
-      zd t3{xr - 0.5 * b.im, 0.5 * b.re};
-      zd t4 = zmul(t1, t3);
-      h.re -= t4.re;
-      h.im -= t4.im;
-      XT o;
-      from_zd(zd{xr - b.im, b.re}, o);
-      X[(long)n * xsn + (long)w * xsw] = o;
+    for (int n = 2 * wv; n < K; n += 32) {
+      double xa, xb = 0.0;
+      if (x) {
+        xa = x[(long)n * nw + w];
+        if (n + 1 < K) xb = x[(long)(n + 1) * nw + w];
+        bad |= !isfinite(xa) || !isfinite(xb);
+      } else {
+        afqmc_rng::field_pair(src.seed, sub, w, n, xa, xb);
+      }
+      zd o;
+      field_term(vb[(long)n * nw + w], xa, h, o);
+      from_zd(o, X[(long)n * nw + w]);
+      if (n + 1 < K) {
+        field_term(vb[(long)(n + 1) * nw + w], xb, h, o);
+        from_zd(o, X[(long)(n + 1) * nw + w]);
+      }
     }
   }
   if (bad) atomicOr(nonfinite, 1);
@@ -477,7 +548,7 @@ __global__ __launch_bounds__(256) void fields_kernel(int K, int nw, const VT* __
   __syncthreads();
   if (wv == 0 && w < nw) {
     zd t = part[0][lane];
-    for (int q = 1; q < 4; q++) t = zadd(t, part[q][lane]);
+    for (int q = 1; q < 16; q++) t = zadd(t, part[q][lane]);
     hybridW[w] = t;
   }
 }

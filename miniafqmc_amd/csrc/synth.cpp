@@ -66,7 +66,8 @@ template <class F> void parallel_for(long n, F&& f)
   for (auto& x : th) x.join();
 }
 
-enum Stream : uint64_t { S_H = 1, S_PSI_B = 3, S_L = 4, S_X = 5 };
+// stream tags; 5 is the auxiliary fields' (fieldrng.hpp)
+enum Stream : uint64_t { S_H = 1, S_PSI_B = 3, S_L = 4 };
 
 // N orthonormal columns (modified Gram-Schmidt, two passes) of a Gaussian M x N
 void random_orbitals(int M, int N, uint64_t seed, uint64_t stream, cd* Psi)
@@ -359,25 +360,6 @@ int afqmc_synth_get(const afqmc_synth* s, afqmc_synth_view* v)
     v->vakbl.col = s->v_col.data();
     v->vakbl.val = s->v_val.data();
   }
-  return AFQMC_OK;
-}
-
-// Real N(0,1) auxiliary fields for one substep, [K][nwalk] row-major (the
-// layout of the driver's X, miniafqmc_base.cpp:217).  Walker w's column is a
-// function of (seed, substep, w) only, so a walker subset reproduces the
-// full run's columns exactly.
-int afqmc_synth_fields(uint64_t seed, int nchol, int nwalk, int w0, int wtot, long substep, double* x)
-{
-  if (!x || nchol <= 0 || nwalk <= 0 || w0 < 0 || w0 + nwalk > wtot) return AFQMC_ERR_ARG;
-  parallel_for((long)nwalk, [&](long wl) {
-    const uint64_t stream = ((uint64_t)S_X << 56) ^ ((uint64_t)substep << 24) ^ (uint64_t)(w0 + wl);
-    for (int n = 0; n < nchol; n += 2) {
-      double a, b;
-      normal2(seed, stream, (uint64_t)(n / 2), a, b);
-      x[(size_t)n * nwalk + wl] = a;
-      if (n + 1 < nchol) x[(size_t)(n + 1) * nwalk + wl] = b;
-    }
-  });
   return AFQMC_OK;
 }
 

@@ -75,6 +75,15 @@ long orc_halfrotate_cholesky(int M, int N, int K, const double* trial_a, const d
 /* COO -> CSR with (row, col) ordering (SparseMatrix.hpp:295-319) */
 int orc_compress(int nrows, long nnz, int* rows, int* cols, double* vals, int* rowptr);
 
+/* orc_run status for a non-finite auxiliary-field draw (the library's
+ * AFQMC_ERR_NONFINITE); a positive status is LAPACK's info of the first
+ * singular overlap matrix (the library's AFQMC_ERR_SINGULAR). */
+#define ORC_NONFINITE (-6)
+/* OpenMP threads of the walker-parallel loops (n <= 0: all processors);
+ * results do not depend on the count. */
+void orc_set_threads(int n);
+int orc_get_threads(void);
+
 /* full driver loop (miniafqmc_base.cpp:213-349), aux fields from x_all
  * [nsteps*nsubsteps][K][nw]; spt_rp == NULL selects the non-transposed
  * branch (full G and T(Spvn) for the bias, miniafqmc_base.cpp:267-276).

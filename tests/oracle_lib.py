@@ -23,6 +23,17 @@ def _load():
 
 lib = _load()
 
+ORC_NONFINITE = -6  # orc_run status of a non-finite field draw (AFQMC_ERR_NONFINITE)
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's walker-parallel loops (n <= 0: all)."""
+    lib.orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib.orc_get_threads()
+
 
 def dp(a):
     assert a.flags["C_CONTIGUOUS"]

@@ -93,3 +93,78 @@ def test_config_512_orbitals_dense(require_gpu):
     say(f"walker 0 after {nsub} substeps: fp64_dense rel err {e64:.2e}, mixed_bf16 {e16:.2e}")
     assert e64 < 1e-9
     assert e16 < 1e-4
+
+
+# Stated bounds of the split-BF16 path at 512 orbitals / 128+128 electrons /
+# 2048 vectors (energy level 2, exchange chunks of 2048, dense exchange
+# block), one substep from the trial with shared fields, against the FP64
+# oracle on walkers 0-1.  Relative errors are max |GPU - oracle| / max |oracle|
+# over the walker's array.
+MIXED_BOUNDS = {"G": 2e-5, "vbias": 2e-5, "X": 2e-5, "vHS": 2e-5, "hybridW": 2e-5, "W": 1e-4}
+MIXED_DE_L = 1e-5  # Ha, |dE_L| target of the mixed path
+
+
+@pytest.mark.timeout(900)
+def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
+    """The benchmark's configuration (split-BF16 contractions and propagator,
+    FP64 LU/overlaps/energy sums, FP32 exchange in chunks of 2048, fields
+    drawn on the device) against the FP64 oracle, and the FP64 dense path
+    against the oracle at 1e-10: one substep from the trial with the same
+    draws (walkers 0-1 of the 1024), every phase of the substep and the
+    local energy of the propagated walkers (afqmc_sys.hpp:95-133,
+    vbias.hpp:45-83, vHS.hpp:43-91, energy.hpp:63-102)."""
+    s, p = m.preset_system("N128_M512")
+    nw, seed, K = p["nwalk"], p["seed"], s.nchol
+    M, N = s.nmo, s.naea
+    gpu = {}
+    for mode in ("mixed_bf16", "fp64_dense"):
+        say(f"config 512 orbitals: GPU {mode}, {nw} walkers, one substep from the trial")
+        g = m.AfqmcSys(s, nw, mode, energy_precision=2)
+        g.set_exchange_chunk(2048)
+        g.device_fields(seed)
+        g.init_walkers()
+        g.calculate_mixed_density_matrix()
+        g.substep(0, False, s.dt)
+        r = {"G": g.get(nat.BUF_G)[:, :2], "vbias": g.get(nat.BUF_VBIAS)[:, :2], "X": g.get(nat.BUF_X)[:, :2],
+             "vHS": g.get(nat.BUF_VHS)[:, :2], "hybridW": g.get(nat.BUF_HYBRIDW)[:2], "W": g.get(nat.BUF_W)[:2]}
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        r["E"] = g.W_data[:2, 0]
+        r["terms"] = g.energy_terms()[:2]
+        gpu[mode] = r
+        g.close()
+    say("config 512 orbitals: oracle on walkers 0-1")
+    W0 = np.zeros((2, 2, M, N), complex)
+    W0[:, 0] = s.trial_alpha.conj()
+    W0[:, 1] = s.trial_beta.conj()
+    Wd = np.zeros((2, 8), complex)
+    Wd[:, 1] = 1.0
+    G0, info = orc.calculate_mixed_density_matrix(s, W0, Wd)
+    assert info == 0
+    spT = orc.halfrotate(s, cutoff=-1.0)  # dense modes keep every half-rotated entry
+    vb = orc.get_vbias(spT, G0)
+    x = m.fields(seed, K, 2, 0, w0=0, wtot=nw)
+    X, hyb = orc.fields(vb, x)
+    vhs = orc.get_vhs(s.spvn, X)
+    W1 = orc.propagate(s, W0, vhs)
+    Wd1 = Wd.copy()
+    G1, info = orc.calculate_mixed_density_matrix(s, W1, Wd1)
+    assert info == 0
+    Lt = np.ascontiguousarray(spT.dense() / np.sqrt(s.dt))
+    del spT
+    say("config 512 orbitals: oracle local energy (Cholesky form)")
+    orc.energy_cholesky(s, Wd1, G1, Lt)
+    ref = {"G": G0, "vbias": vb, "X": X, "vHS": vhs, "hybridW": hyb, "W": W1}
+    for mode, bounds in (("fp64_dense", None), ("mixed_bf16", MIXED_BOUNDS)):
+        r = gpu[mode]
+        for k_, v in ref.items():
+            for w in range(2):
+                e = rel(r[k_][w] if k_ in ("hybridW", "W") else r[k_][:, w], v[w] if k_ in ("hybridW", "W") else v[:, w])
+                say(f"{mode} walker {w} {k_}: rel err {e:.2e}")
+                assert e < (1e-10 if bounds is None else bounds[k_]), (mode, k_, w, e)
+        dE = np.abs(r["E"] - Wd1[:, 0])
+        say(f"{mode} E_L: GPU {r['E'].real} oracle {Wd1[:, 0].real} |dE_L| {dE} Ha; terms {r['terms'].real}")
+        if bounds is None:
+            assert np.all(dE <= 1e-10 * np.abs(Wd1[:, 0]))
+        else:
+            assert np.all(dE < MIXED_DE_L)
