@@ -32,6 +32,10 @@ PEAKS = {  # MI355X published dense peaks, TFLOP/s
     "fp32": 157.3,
     "bf16": 2500.0,
 }
+# the headline metric: the mixed-precision step at 512 orbitals, 128+128
+# electrons and 2048 Cholesky vectors, compared with the FP64 path at a
+# per-walker local-energy tolerance of 1e-5 Ha
+HEADLINE_METRIC = "walker-steps/s at M=512,N=128,nchol=2048; mixed vs FP64 at |\u0394E_L|<1e-5 Ha"
 MODE_PEAK = {"fp64_csr": "fp64", "fp64_dense": "fp64", "mixed_fp32": "fp32", "mixed_bf16": "bf16"}
 MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128", "mixed_bf16": "bf16x3+c128"}
 
@@ -75,6 +79,47 @@ def kernel_flops(name, M, N, K, nw):
         "gj_inverse": 2 * nw * N * N * N,
     }.get(name)
     return None if c is None else 8.0 * c
+
+
+def kernel_flops_total(name, launches, M, N, K, nw, n_energy):
+    """Algorithmic real flops of all `launches` of a call site.  The exchange
+    GEMM's launches differ in size (column chunks of the lower triangle), so
+    its total is counted per energy evaluation: both spins, every walker,
+    the NM(NM+1)/2 entries of the triangular block of the same-spin exchange
+    matrix, one complex multiply-add each."""
+    if name == "energy_vx_gemm":
+        NM = N * M
+        return 8.0 * n_energy * 2 * nw * NM * (NM + 1) / 2
+    f = kernel_flops(name, M, N, K, nw)
+    return None if f is None else f * launches
+
+
+def kernel_peak(name, mode, energy_level):
+    """Peak (TFLOP/s) of the unit a call site runs on in a precision mode."""
+    if mode in ("fp64_csr", "fp64_dense") or name in ("ovlp_gemm", "gj_inverse"):
+        return "fp64", PEAKS["fp64"]
+    if name == "energy_vx_gemm":
+        u = {0: MODE_PEAK[mode], 1: MODE_PEAK[mode], 2: "fp32"}.get(energy_level, "fp64")
+        return u, PEAKS[u]
+    if name == "energy_v_gemm" and energy_level >= 1:
+        return "fp64", PEAKS["fp64"]
+    return MODE_PEAK[mode], PEAKS[MODE_PEAK[mode]]
+
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else avail
+    return model, avail, threads
 
 
 def bench_multi(args, m):
@@ -223,7 +268,6 @@ def main():
     spe = args.substeps_per_energy or args.steps
     if args.steps % spe:
         sys.exit("bench.py: --steps must be a multiple of --substeps-per-energy")
-    x = m.fields_block(p["seed"], K, nw, 0, nsub)
     print(f"[bench] system M={M} N={N} K={K} nnz={s.spvn.nnz} walkers={nw} generated in {t_gen:.1f}s", flush=True)
 
     probe_steps = (1, 10)
@@ -256,7 +300,10 @@ def main():
         g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level)
         g.set_exchange_chunk(args.exchange_chunk)
         g.set_direct_lds(args.direct_lds)
-        g.stage_fields(x)
+        # the auxiliary fields are drawn on the device inside each substep
+        # (the reference's generate_normal in the step loop), the same
+        # counter-based draws the oracle is given on the host
+        g.device_fields(p["seed"])
         if args.tune:
             tt = time.perf_counter()
             g.tune(s.dt)
@@ -313,9 +360,11 @@ def main():
     for k_, (n_, ms) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         print(f"[bench]   {k_:20s} {n_:6d} launches {ms:10.2f} ms total {ms / max(n_, 1):9.3f} ms avg", flush=True)
 
-    # roofline of the dominant kernel with a known algorithmic count
+    # roofline of the dominant kernel with a known algorithmic count: the
+    # call site with the largest total time in the timed block among those
+    # whose algorithmic work is known; the next two follow in "kernels"
     roof = None
-    timed = [(k_, v) for k_, v in stats.items() if kernel_flops(k_, M, N, K, nw) is not None]
+    n_energy = 0 if args.no_energy else args.steps // spe
     csr = [(k_, v) for k_, v in stats.items() if k_ in ("csrmm_vhs", "csrmm_vbias")]
     if args.mode == "fp64_csr" and csr:
         # the reference's sparse products: report HBM bandwidth against the
@@ -329,15 +378,29 @@ def main():
                 "frac": round(ach / 8000.0, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
                 "avg_ms": round(ms / nl, 4), "bytes_per_launch": nbytes,
                 "fp64_tflops": round(fl / avg_s / 1e12, 2)}
-    elif timed:
-        name, (nl, ms) = max(timed, key=lambda kv: kv[1][1])
-        avg_s = ms / nl / 1e3
-        flops = kernel_flops(name, M, N, K, nw)
-        ach = flops / avg_s / 1e12
-        pk = PEAKS[MODE_PEAK[args.mode] if name not in ("ovlp_gemm", "gc_gemm", "gj_inverse") else "fp64"]
-        roof = {"kernel": name, "bound": "mfma", "achieved": round(ach, 2), "peak": pk, "unit": "TFLOP/s",
-                "frac": round(ach / pk, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
-                "avg_ms": round(ms / nl, 4), "flops_per_launch": flops}
+    else:
+        rows = []
+        for k_, (nl, ms) in stats.items():
+            fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy)
+            if fl is None or nl == 0 or ms <= 0:
+                continue
+            unit, pk = kernel_peak(k_, args.mode, args.energy_level)
+            ach = fl / (ms / 1e3) / 1e12
+            rows.append({"kernel": k_, "achieved": round(ach, 2), "peak": pk, "peak_unit": unit,
+                         "frac": round(ach / pk, 4), "ms_total": round(ms, 2), "launches": nl,
+                         "avg_ms": round(ms / nl, 4), "flops_per_launch": fl / nl})
+        rows.sort(key=lambda r: -r["ms_total"])
+        if rows:
+            top = rows[0]
+            roof = {"kernel": top["kernel"], "bound": "mfma", "achieved": top["achieved"], "peak": top["peak"],
+                    "unit": "TFLOP/s", "frac": top["frac"],
+                    "traffic": MEASURED_TRAFFIC.get((top["kernel"], args.workload, args.mode)),
+                    "avg_ms": top["avg_ms"], "flops_per_launch": top["flops_per_launch"],
+                    "kernels": [{k_: r[k_] for k_ in ("kernel", "achieved", "peak", "peak_unit", "frac", "ms_total",
+                                                      "launches")} for r in rows[:3]]}
+            for r in rows:
+                print(f"[bench] roofline {r['kernel']:16s} {r['achieved']:8.1f} TFLOP/s of {r['peak']:.1f} "
+                      f"({r['peak_unit']}) = {r['frac']:.3f}, {r['ms_total']:.1f} ms", flush=True)
 
     extra = {}
     if args.fp64 and args.mode != "fp64_dense":
@@ -383,21 +446,39 @@ def main():
     if args.cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as orc
-        cw = args.cpu_walkers
+        model, avail, threads = cpu_info()
         tr = time.time()
+        orc.set_threads(threads)
         spT = orc.halfrotate(s, cutoff=1e-6)
-        print(f"[bench] oracle half rotation {time.time() - tr:.1f}s (setup, untimed)", flush=True)
-        xs = x[:1, :, :cw].copy()
-        eo, Wo, Wdo, eso, tm, info = orc.run(s, cw, 1, 1, 1 << 30, xs, energy_mode=2, spT=spT)
-        tcpu = tm[0]
-        cpu = {"value": round(cw / tcpu, 4), "unit": "walker-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{cw} walker(s) x 1 substep of the same workload (no energy), "
-                         f"{tcpu:.1f}s on 1 host core; oracle = FP64 C++ restatement, CSR SpvnT/Spvn"}
-        print(f"[bench] cpu oracle: {tcpu:.2f}s per {cw} walker-substep(s) -> {cw / tcpu:.3f} walker-steps/s",
+        print(f"[bench] oracle half rotation {time.time() - tr:.1f}s on {threads} threads (setup, untimed)",
               flush=True)
 
+        def cpu_leg(cw, nthreads):
+            # walkers 0..cw-1 of the same run: the counter-based draws give
+            # them the fields they have in the full GPU run
+            xs = m.fields(p["seed"], K, cw, 0, w0=0, wtot=nw)[None].copy()
+            orc.set_threads(nthreads)
+            eo, Wo, Wdo, eso, tm, info = orc.run(s, cw, 1, 1, 1 << 30, xs, energy_mode=2, spT=spT)
+            return tm[0]
+
+        cw1, cwa = args.cpu_walkers, max(args.cpu_walkers, 4 * threads)
+        t1 = cpu_leg(cw1, 1)
+        ta_ = cpu_leg(cwa, threads)
+        orc.set_threads(threads)
+        cpu = {"value": round(cwa / ta_, 4), "unit": "walker-steps/s", "cores": threads, "kind": "port",
+               "cpu_model": model, "cpus_available": avail,
+               "sample": f"{cwa} walkers x 1 substep of the same workload (no energy), {ta_:.1f}s on {threads} "
+                         f"threads (OpenMP over walkers); oracle = FP64 C++ restatement of the reference, "
+                         f"CSR SpvnT/Spvn",
+               "single_thread": {"value": round(cw1 / t1, 4), "cores": 1,
+                                 "sample": f"{cw1} walkers x 1 substep, {t1:.1f}s on 1 thread (the reference's "
+                                           f"serial execution model)"}}
+        print(f"[bench] cpu oracle: {cw1 / t1:.3f} walker-steps/s on 1 thread, {cwa / ta_:.3f} on {threads} "
+              f"threads ({model})", flush=True)
+
     line = {
-        "metric": f"walker-steps/s (M={M}, N={N}+{N}, nchol={K}, {nw} walkers)",
+        "metric": (HEADLINE_METRIC if args.workload == "N128_M512" and args.mode == "mixed_bf16"
+                   else f"walker-steps/s (M={M}, N={N}+{N}, nchol={K}, {nw} walkers, {args.mode})"),
         "value": round(value, 2),
         "unit": "walker-steps/s",
         "n_gpus": 1,

@@ -647,20 +647,43 @@ double weights(int nw, double dt, const cd* hybridW, cd* Wd, double Eshift)
 void csr_transpose(int rows, int cols, const int* rp, const int* ci, const cd* v, std::vector<int>& trp,
                    std::vector<int>& tci, std::vector<cd>& tv)
 {
-  long nnz = rp[rows] - rp[0];
+  // Rows are split into one contiguous range per thread; each thread counts
+  // its entries per column, the counts are prefix-summed column-major over
+  // the threads, and each thread scatters its range in row order, so every
+  // column keeps its entries in row order (the serial result).
+  const long nnz = rp[rows] - rp[0];
+  const int nt = std::max(1, std::min(omp_get_max_threads(), rows));
+  std::vector<long> cnt((size_t)nt * cols, 0);
+  auto lo = [&](int t) { return (int)((long)rows * t / nt); };
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+  for (int t = 0; t < nt; t++) {
+    long* ct = cnt.data() + (size_t)t * cols;
+    for (long p = rp[lo(t)] - rp[0]; p < rp[lo(t + 1)] - rp[0]; p++) ct[ci[p]]++;
+  }
   trp.assign(cols + 1, 0);
-  for (long p = 0; p < nnz; p++) trp[ci[p] + 1]++;
-  for (int c = 0; c < cols; c++) trp[c + 1] += trp[c];
+  long run = 0;
+  for (int c = 0; c < cols; c++) {
+    trp[c] = (int)run;
+    for (int t = 0; t < nt; t++) {
+      const long n = cnt[(size_t)t * cols + c];
+      cnt[(size_t)t * cols + c] = run;  // thread t's first slot in column c
+      run += n;
+    }
+  }
+  trp[cols] = (int)run;
   tci.resize(nnz);
   tv.resize(nnz);
-  std::vector<int> pos(trp.begin(), trp.end() - 1);
-  for (int r = 0; r < rows; r++)
-    for (int p = rp[r] - rp[0]; p < rp[r + 1] - rp[0]; p++) {
-      int c = ci[p];
-      tci[pos[c]] = r;
-      tv[pos[c]] = v[p];
-      pos[c]++;
-    }
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+  for (int t = 0; t < nt; t++) {
+    long* pos = cnt.data() + (size_t)t * cols;
+    for (int r = lo(t); r < lo(t + 1); r++)
+      for (long p = rp[r] - rp[0]; p < rp[r + 1] - rp[0]; p++) {
+        const int c = ci[p];
+        tci[pos[c]] = r;
+        tv[pos[c]] = v[p];
+        pos[c]++;
+      }
+  }
 }
 
 } // namespace
@@ -872,17 +895,26 @@ int orc_branch(int M, int N, int nw, double u, double* W_, double* Wd_, int* par
 /* base::halfrotate_cholesky (rotate.hpp:65-169).  The reference counts in a
  * first pass and fills in a second with identical arithmetic; one pass that
  * keeps the kept entries gives the same matrix. */
-long orc_halfrotate_cholesky(int M, int N, int K, const double* ta_, const double* tb_, const int* rp,
-                             const int* col, const double* val, double cutoff, int* out_rp, int* out_col,
-                             double* out_val)
+struct HalfRotation {
+  int K;
+  std::vector<std::vector<int>> rc;
+  std::vector<std::vector<cd>> rv;
+};
+
+static HalfRotation* halfrotate(int M, int N, int K, const double* ta_, const double* tb_, const int* rp,
+                                const int* col, const double* val, double cutoff)
 {
+  auto* h = new HalfRotation;
+  h->K = K;
   const cd* ta = CCZ(ta_);
   const cd* tb = CCZ(tb_);
   std::vector<int> trp, tci;
   std::vector<cd> tv;
   csr_transpose(M * M, K, rp, col, CCZ(val), trp, tci, tv);
-  std::vector<std::vector<int>> rc(K);
-  std::vector<std::vector<cd>> rv(K);
+  auto& rc = h->rc;
+  auto& rv = h->rv;
+  rc.resize(K);
+  rv.resize(K);
 #pragma omp parallel
   {
     std::vector<cd> C((size_t)N * M), An((size_t)M * M);
@@ -922,20 +954,53 @@ long orc_halfrotate_cholesky(int M, int N, int K, const double* ta_, const doubl
       }
     }
   }
+  return h;
+}
+
+static long halfrot_nnz(const HalfRotation* h)
+{
   long nz = 0;
-  for (int i = 0; i < K; i++) nz += (long)rc[i].size();
-  if (!out_rp) return nz;
+  for (int i = 0; i < h->K; i++) nz += (long)h->rc[i].size();
+  return nz;
+}
+
+static void halfrot_fill(const HalfRotation* h, int* out_rp, int* out_col, double* out_val)
+{
   long p = 0;
-  for (int i = 0; i < K; i++) {
+  for (int i = 0; i < h->K; i++) {
     out_rp[i] = (int)p;
-    for (size_t j = 0; j < rc[i].size(); j++, p++) {
-      out_col[p] = rc[i][j];
-      out_val[2 * p] = rv[i][j].real();
-      out_val[2 * p + 1] = rv[i][j].imag();
+    for (size_t j = 0; j < h->rc[i].size(); j++, p++) {
+      out_col[p] = h->rc[i][j];
+      out_val[2 * p] = h->rv[i][j].real();
+      out_val[2 * p + 1] = h->rv[i][j].imag();
     }
   }
-  out_rp[K] = (int)p;
+  out_rp[h->K] = (int)p;
+}
+
+long orc_halfrotate_cholesky(int M, int N, int K, const double* ta, const double* tb, const int* rp, const int* col,
+                             const double* val, double cutoff, int* out_rp, int* out_col, double* out_val)
+{
+  HalfRotation* h = halfrotate(M, N, K, ta, tb, rp, col, val, cutoff);
+  const long nz = halfrot_nnz(h);
+  if (out_rp) halfrot_fill(h, out_rp, out_col, out_val);
+  delete h;
   return nz;
+}
+
+void* orc_halfrotate_begin(int M, int N, int K, const double* ta, const double* tb, const int* rp, const int* col,
+                           const double* val, double cutoff, long* nnz_out)
+{
+  HalfRotation* h = halfrotate(M, N, K, ta, tb, rp, col, val, cutoff);
+  *nnz_out = halfrot_nnz(h);
+  return h;
+}
+
+void orc_halfrotate_take(void* handle, int* out_rp, int* out_col, double* out_val)
+{
+  auto* h = static_cast<HalfRotation*>(handle);
+  if (out_rp) halfrot_fill(h, out_rp, out_col, out_val);
+  delete h;
 }
 
 /* SparseMatrix::compress (SparseMatrix.hpp:295-319): sort by (row, col),

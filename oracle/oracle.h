@@ -72,6 +72,12 @@ int orc_branch(int M, int N, int nw, double u, double* W, double* Wd, int* paren
 long orc_halfrotate_cholesky(int M, int N, int K, const double* trial_a, const double* trial_b,
                              const int* rowptr, const int* col, const double* val,
                              double cutoff, int* out_rowptr, int* out_col, double* out_val);
+/* the same in one computation: begin returns a handle and nnz, take fills
+ * arrays of that size (or only releases the handle when out_rowptr is NULL) */
+void* orc_halfrotate_begin(int M, int N, int K, const double* trial_a, const double* trial_b,
+                           const int* rowptr, const int* col, const double* val, double cutoff,
+                           long* nnz_out);
+void orc_halfrotate_take(void* handle, int* out_rowptr, int* out_col, double* out_val);
 /* COO -> CSR with (row, col) ordering (SparseMatrix.hpp:295-319) */
 int orc_compress(int nrows, long nnz, int* rows, int* cols, double* vals, int* rowptr);
 

@@ -17,6 +17,8 @@ def _load():
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle")])
     lib = C.CDLL(LIB)
     lib.orc_halfrotate_cholesky.restype = C.c_long
+    lib.orc_halfrotate_begin.restype = C.c_void_p
+    lib.orc_halfrotate_take.restype = None
     lib.orc_product.restype = None
     return lib
 
@@ -151,11 +153,12 @@ def halfrotate(sysm, cutoff=1e-6):
     M, N, K = sysm.nmo, sysm.naea, sysm.nchol
     sp = sysm.spvn
     args = (M, N, K, dp(sysm.trial_alpha), dp(sysm.trial_beta), ip(sp.rowptr), ip(sp.col), dp(sp.val), C.c_double(cutoff))
-    nnz = lib.orc_halfrotate_cholesky(*args, None, None, None)
+    nnz = C.c_long(0)
+    h = lib.orc_halfrotate_begin(*args, C.byref(nnz))
     rp = np.zeros(K + 1, np.int32)
-    col = np.zeros(nnz, np.int32)
-    val = np.zeros(nnz, np.complex128)
-    lib.orc_halfrotate_cholesky(*args, ip(rp), ip(col), dp(val))
+    col = np.zeros(nnz.value, np.int32)
+    val = np.zeros(nnz.value, np.complex128)
+    lib.orc_halfrotate_take(C.c_void_p(h), ip(rp), ip(col), dp(val))
     return Csr(K, 2 * N * M, rp, col, val)
 
 
