@@ -163,6 +163,29 @@ int afqmc_copy(afqmc_ctx* ctx, int buffer, int to_device, double* host);
 int afqmc_mixed_density_matrix(afqmc_ctx* ctx);
 /* base::get_vbias(SpvnT, Gc, vbias, transposed=true) (src/AFQMC/vbias.hpp:45-83) */
 int afqmc_vbias(afqmc_ctx* ctx);
+
+/* No counterpart in the reference: an allocation-free variant of
+ * afqmc_mixed_density_matrix for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It needs no context, allocates nothing and reads nothing back
+ * to the host; it runs the same three kernels (overlap product, batched
+ * inverse + determinant, Gc product) on `stream` (a hipStream_t, NULL for the
+ * null stream) in FP64.  All pointers are DEVICE memory:
+ *   trialwfn [2][M][N]  conj(Psi) per spin, as afqmc_upload_system's
+ *                       trial_alpha, trial_beta;
+ *   W        [nwalk][2][M][N] walkers;
+ *   Gc       [2*N*M][nwalk]   output, AFQMC_BUF_G's layout;
+ *   ovlp     [nwalk][2]       output, det(W_s^T trialwfn_s) (W_data[w][2+s]).
+ * The overlap matrices (nwalk*2*N*N complex doubles) live in a static device
+ * array of AFQMC_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * exits; larger requests return AFQMC_ERR_SHAPE (N = 128: 128 walkers per
+ * call), N > 128 returns AFQMC_ERR_UNSUPPORTED.  Every call on one GPU
+ * shares that array, so calls that overlap in time on one GPU (other
+ * streams or threads) race.  An exactly singular overlap matrix gives det 0
+ * and zero Gc columns (there is no status: nothing is read back). */
+#define AFQMC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int afqmc_mixed_density_matrix_noalloc(int nmo, int naea, int nwalk, const double* trialwfn, const double* W,
+                                       double* Gc, double* ovlp, void* stream);
 /* Replaces afqmc_sys::calculate_mixed_density_matrix(W, W_data, G, false)
  * (src/AFQMC/afqmc_sys.hpp:95-116 with mixed_density_matrix.hpp:93-101):
  * full G [2*M*M][nwalk] = conj(A) T1^-1 W^T per spin, and the overlaps.

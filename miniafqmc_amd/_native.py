@@ -22,6 +22,9 @@ STATUS = {
     8: "AFQMC_ERR_UNSUPPORTED",
 }
 
+# bytes of the static device scratch array of afqmc_mixed_density_matrix_noalloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 MODES = {"fp64_csr": 0, "fp64_dense": 1, "mixed_fp32": 2, "mixed_bf16": 3}
 
 BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS, BUF_GFULL = range(9)
@@ -32,7 +35,8 @@ BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS, BUF
 EXPORTED = [
     "afqmc_ctx_create", "afqmc_ctx_destroy", "afqmc_ctx_error", "afqmc_sync", "afqmc_set_option",
     "afqmc_upload_system", "afqmc_get_spvnT", "afqmc_init_walkers", "afqmc_copy",
-    "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_mixed_density_matrix_full", "afqmc_vbias_full",
+    "afqmc_mixed_density_matrix", "afqmc_vbias", "afqmc_mixed_density_matrix_noalloc",
+    "afqmc_mixed_density_matrix_full", "afqmc_vbias_full",
     "afqmc_fields", "afqmc_vhs",
     "afqmc_propagate", "afqmc_save_overlaps", "afqmc_calculate_overlaps",
     "afqmc_weights", "afqmc_branch", "afqmc_orthogonalize", "afqmc_energy", "afqmc_stage_fields",
@@ -104,6 +108,7 @@ def _load():
         "afqmc_copy": (I, [P, I, I, D]),
         "afqmc_mixed_density_matrix": (I, [P]),
         "afqmc_vbias": (I, [P]),
+        "afqmc_mixed_density_matrix_noalloc": (I, [I, I, I, P, P, P, P, P]),
         "afqmc_mixed_density_matrix_full": (I, [P]),
         "afqmc_vbias_full": (I, [P]),
         "afqmc_fields": (I, [P, D, L]),

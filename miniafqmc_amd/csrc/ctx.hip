@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -466,34 +467,41 @@ int csrmm_n(afqmc_ctx* c, const char* name, int nrows, int K, long long nnz, con
   });
 }
 
-// Batched inverse + determinant of the nw*2 overlap matrices in c->T1.
-int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
+// Batched inverse + determinant of the nw*2 overlap matrices T (in place
+// when Tinv == T); det of matrix b at det[(b/2)*dq + (b%2)*dr], a count of
+// singular matrices in *flags and the determinants in dc when those are set.
+int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, int* flags, zd* dc)
 {
   const int N = c->N;
   const long batch = 2L * c->nw;
   const long ms = (long)N * N;
-  zd* dc = c->dcache;
   if (N <= 16)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<16, 4>), dim3(batch), dim3(64), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags, dc);
+                         flags, dc);
     });
   if (N <= 32)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<32, 4>), dim3(batch), dim3(128), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags, dc);
+                         flags, dc);
     });
   if (N <= 64)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<64, 4>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         c->flags, dc);
+                         flags, dc);
     });
   if (N <= 128)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<128, 4>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N,
-                         ms, c->flags, dc);
+                         ms, flags, dc);
     });
   return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 128 electrons per spin is not supported by the batched inverse");
+}
+
+// ... of the context's overlap matrices, with its flags and determinant cache
+int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
+{
+  return gj_inverse_into(c, T, Tinv, det, dq, dr, c->flags, c->dcache);
 }
 
 // T1[w][s] = W[w][s]^T * trial_s  (N x N, k = M), all walkers and spins
@@ -2299,6 +2307,106 @@ extern "C" int afqmc_zgemm(int precision, char opA, char opB, int m, int n, int 
   for (void* p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free mixed density matrix.  The overlap matrices (the only
+// memory the three kernels need beyond the caller's inputs and outputs) live
+// in a file-scope device array, one copy per GPU, held until the process
+// ends.  Host code never passes the array's name to a kernel: its device
+// address is looked up once per GPU with hipGetSymbolAddress (with that GPU
+// current) and kept per device.
+namespace {
+static __device__ __align__(256) unsigned char scratch[AFQMC_NOALLOC_SCRATCH_BYTES];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of this GPU's scratch array (nullptr on failure)
+void* scratch_address(int dev)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+} // namespace
+
+extern "C" int afqmc_mixed_density_matrix_noalloc(int nmo, int naea, int nwalk, const double* trialwfn,
+                                                  const double* W, double* Gc, double* ovlp, void* stream)
+{
+  if (!trialwfn || !W || !Gc || !ovlp || nmo <= 0 || naea <= 0 || nwalk <= 0) return AFQMC_ERR_ARG;
+  if (naea > nmo) return AFQMC_ERR_SHAPE;
+  if (naea > 128) return AFQMC_ERR_UNSUPPORTED;
+  const long M = nmo, N = naea, nw = nwalk;
+  const size_t need = sizeof(zd) * 2 * (size_t)nw * N * N;
+  if (need > (size_t)AFQMC_NOALLOC_SCRATCH_BYTES) return AFQMC_ERR_SHAPE;
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return AFQMC_ERR_HIP;
+  zd* T1 = (zd*)scratch_address(dev);
+  if (!T1) return AFQMC_ERR_HIP;
+  // a context on the stack only carries the stream and sizes into the
+  // launchers below (no member of it allocates)
+  afqmc_ctx tmp;
+  afqmc_ctx* c = &tmp;
+  c->stream = (hipStream_t)stream;
+  c->M = nmo;
+  c->N = naea;
+  c->nw = nwalk;
+  // T1[w][s] = W[w][s]^T trialwfn_s; every entry is written (beta = 0)
+  ZGemmArgs a = gargs();
+  a.A = W;
+  a.sam = 1;
+  a.sak = N;
+  a.sa1 = 2 * M * N;
+  a.sa2 = M * N;
+  a.B = trialwfn;
+  a.sbn = 1;
+  a.sbk = N;
+  a.sb1 = 0;
+  a.sb2 = M * N;
+  a.C = T1;
+  a.scm = N;
+  a.scn = 1;
+  a.sc1 = 2 * N * N;
+  a.sc2 = N * N;
+  a.bdiv = 2;
+  a.m = naea;
+  a.n = naea;
+  a.k = nmo;
+  int r = zgemm_run<CP_F64, double, double, double, false, false>(c, "ovlp_gemm", a, 2 * nwalk,
+                                                                   default_cfg<CP_F64>(a));
+  // in-place inverse and determinant (no status flag: nothing is read back)
+  if (r == AFQMC_OK) r = gj_inverse_into(c, T1, T1, (zd*)ovlp, 2, 1, nullptr, nullptr);
+  if (r == AFQMC_OK) {
+    // Gc[(s N + a) M + k][w] = sum_b T1inv(a, b) W(k, b)
+    ZGemmArgs g = gargs();
+    g.A = T1;
+    g.sam = N;
+    g.sak = 1;
+    g.sa1 = 2 * N * N;
+    g.sa2 = N * N;
+    g.B = W;
+    g.sbn = N;
+    g.sbk = 1;
+    g.sb1 = 2 * M * N;
+    g.sb2 = M * N;
+    g.bdiv = 2;
+    g.m = naea;
+    g.n = nmo;
+    g.k = naea;
+    g.C = Gc;
+    g.scm = M * nw;
+    g.scn = nw;
+    g.sc1 = 1;
+    g.sc2 = N * M * nw;
+    r = zgemm_run<CP_F64, double, double, double, true, true>(c, "gc_gemm", g, 2 * nwalk, default_cfg<CP_F64>(g));
+  }
   c->stream = nullptr;
   return r;
 }

@@ -88,3 +88,13 @@ def test_python_constants_match_header():
         hdr = {k[len("AFQMC_"):]: v for k, v in enum.items() if k.startswith(prefix)}
         py = {k: getattr(nat, k) for k in dir(nat) if k.startswith(prefix[len("AFQMC_"):])}
         assert hdr == py, (prefix, hdr, py)
+
+
+def test_noalloc_scratch_size_matches_header():
+    """The Python mirror's size of the allocation-free entry point's scratch
+    array is the header's AFQMC_NOALLOC_SCRATCH_BYTES."""
+    import miniafqmc_amd._native as nat
+    text = open(os.path.join(ROOT, "include", "afqmc_hip.h")).read()
+    mm = re.search(r"#define\s+AFQMC_NOALLOC_SCRATCH_BYTES\s+\((\d+)u\s*<<\s*(\d+)\)", text)
+    assert mm, "AFQMC_NOALLOC_SCRATCH_BYTES not found"
+    assert int(mm.group(1)) << int(mm.group(2)) == nat.NOALLOC_SCRATCH_BYTES
