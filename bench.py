@@ -99,7 +99,9 @@ def kernel_peak(name, mode, energy_level):
     if mode in ("fp64_csr", "fp64_dense") or name in ("ovlp_gemm", "gj_inverse"):
         return "fp64", PEAKS["fp64"]
     if name == "energy_vx_gemm":
-        u = {0: MODE_PEAK[mode], 1: MODE_PEAK[mode], 2: "fp32"}.get(energy_level, "fp64")
+        # level 2: three-piece split-BF16 in the split-BF16 mode, FP32 otherwise
+        u = {0: MODE_PEAK[mode], 1: MODE_PEAK[mode], 2: "bf16" if mode == "mixed_bf16" else "fp32"}.get(
+            energy_level, "fp64")
         return u, PEAKS[u]
     if name == "energy_v_gemm" and energy_level >= 1:
         return "fp64", PEAKS["fp64"]
@@ -225,9 +227,9 @@ def main():
                          "of all timed substeps")
     ap.add_argument("--tune", type=int, default=1, help="autotune GEMM tiles before the warmup (untimed)")
     ap.add_argument("--graphs", type=int, default=1, help="replay substeps from captured HIP graphs (0: eager)")
-    ap.add_argument("--direct-lds", type=int, default=1,
-                    help="split-BF16 GEMMs stage their split left operand by direct global-to-LDS loads (0: "
-                         "through registers)")
+    ap.add_argument("--direct-lds", type=int, default=2,
+                    help="split-BF16 GEMMs: 2 both split operands through a ring of LDS stages by direct loads, "
+                         "1 the left operand by direct loads into two stages, 0 through registers")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
                          "hardware byte counters)")
@@ -493,7 +495,7 @@ def main():
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
                    "exchange_chunk": args.exchange_chunk if args.energy_level < 3 else None,
-                   "graphs": bool(args.graphs), "direct_lds": bool(args.direct_lds),
+                   "graphs": bool(args.graphs), "direct_lds": args.direct_lds,
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -122,12 +122,20 @@ typedef enum {
    * eight GEMM launches, which measured 2 % faster at 512 orbitals.  Both
    * give the same walkers bit for bit.  May be changed at any time. */
   AFQMC_OPT_FUSED_PROPAGATOR = 5,
-  /* AFQMC_MIXED_BF16: 1 (default) stages the split-BF16 left operand of the
-   * GEMMs (Spvn, P, the walkers' vHS) from global memory straight into LDS
-   * with 16-byte direct loads where its layout allows; 0 passes it through
-   * registers.  Both give the same results bit for bit.  May be changed at
-   * any time.  No counterpart in the reference. */
-  AFQMC_OPT_DIRECT_LDS = 6
+  /* AFQMC_MIXED_BF16: 2 (default) runs the GEMMs whose operands are both
+   * split-BF16 with k contiguous (vHS, the bias product, the Taylor steps
+   * after the first, P U) with both operands staged from global memory
+   * straight into a ring of LDS stages by 16-byte direct loads, and the
+   * others as 1; 1 stages the split-BF16 left operand (and a right one that
+   * allows it) by direct loads into two LDS stages; 0 passes every operand
+   * through registers.  All give the same results bit for bit.  May be
+   * changed at any time.  No counterpart in the reference. */
+  AFQMC_OPT_DIRECT_LDS = 6,
+  /* 0 (default): each GEMM call site runs the tile shape the autotuner
+   * (afqmc_tune) or the size heuristic picked; k > 0 forces candidate k of
+   * the call site's kind on every GEMM (tests cover every shape this way;
+   * results are the same bit for bit).  No counterpart in the reference. */
+  AFQMC_OPT_TILE = 7
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

@@ -160,10 +160,18 @@ class AfqmcSys:
         """Split-BF16 mode: the whole propagator in one kernel (1) or as eight GEMMs (0)."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_FUSED_PROPAGATOR, 1.0 if on else 0.0))
 
-    def set_direct_lds(self, on):
-        """Split-BF16 mode: stage the GEMMs' split left operands by direct
-        global-to-LDS loads (1, default) or through registers (0)."""
-        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DIRECT_LDS, 1.0 if on else 0.0))
+    def set_direct_lds(self, level=2):
+        """Split-BF16 mode: 2 (default, also True) both split operands staged
+        by direct global-to-LDS loads through a ring of LDS stages where they
+        allow it, 1 the split left operand by direct loads into two stages,
+        0 (False) every operand through registers."""
+        level = 2 if level is True else int(level)
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DIRECT_LDS, float(level)))
+
+    def set_tile(self, k):
+        """Force tile candidate k (1-based) of each GEMM's kind; 0 restores
+        the autotuned / heuristic choice."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_TILE, float(k)))
 
     def set_exchange_chunk(self, cols):
         """Dense exchange: accumulate the exchange GEMM over chunks of `cols`

@@ -66,6 +66,21 @@ __device__ inline zb split_z(float re, float im)
   unsigned short lr = f2bf(re - bf2f(hr)), li = f2bf(im - bf2f(hi));
   return zb{(uint32_t)hr | ((uint32_t)hi << 16), (uint32_t)lr | ((uint32_t)li << 16)};
 }
+// Three-piece split-BF16: x = h + m + l per component, h = bf16(x),
+// m = bf16(x - h), l = bf16(x - h - m) (the residuals in FP64), which holds
+// 24 significant bits.  (h, m) is stored as a zb, exactly the two-piece split
+// of x; l goes to a separate array of one word (bf16(re) | bf16(im) << 16)
+// per complex, so two-piece consumers read the same zb array.
+__device__ inline unsigned short d2bf(double x) { return f2bf((float)x); }
+__device__ inline void split3_z(double re, double im, zb& hm, uint32_t& l)
+{
+  const unsigned short hr = d2bf(re), hi = d2bf(im);
+  const double rr = re - (double)bf2f(hr), ri = im - (double)bf2f(hi);
+  const unsigned short mr = d2bf(rr), mi = d2bf(ri);
+  const unsigned short lr = d2bf(rr - (double)bf2f(mr)), li = d2bf(ri - (double)bf2f(mi));
+  hm = zb{(uint32_t)hr | ((uint32_t)hi << 16), (uint32_t)mr | ((uint32_t)mi << 16)};
+  l = (uint32_t)lr | ((uint32_t)li << 16);
+}
 __device__ inline zd to_zd(zd v) { return v; }
 __device__ inline zd to_zd(zf v) { return zd{(double)v.re, (double)v.im}; }
 __device__ inline zd to_zd(zb v)

@@ -13,6 +13,7 @@
 #include "afqmc_hip.h"
 #include "common.hpp"
 #include "gemm.hpp"
+#include "gemm_b16.hpp"
 #include "kernels.hpp"
 #include "propagate.hpp"
 
@@ -91,6 +92,7 @@ struct afqmc_ctx {
   zd* Wd = nullptr;
   zd* T1 = nullptr;       // [nw][2][N][N] overlap matrices, then their inverses
   zd* dcache = nullptr;   // [nw][2] determinants of the last inversion
+  zd* gjwork = nullptr;   // N > 128: [nw][2][N][N] working matrices of the batched inverse
   bool t1inv_valid = false;  // T1 holds the inverses for the current walkers
   bool g_fresh = false;      // G was built from T1 and W as they are now
   zd* Wbr = nullptr;         // population control: gathered walkers [nw][2][M][N]
@@ -126,6 +128,12 @@ struct afqmc_ctx {
   // same-spin exchange block of Vakbl, dense [2][NM][NM] (scaled by dt), in
   // the contraction precision; null when it does not fit in device memory
   void* Vx = nullptr;
+  // split-BF16 mode, energy level 2: Vx as three-piece split-BF16 (the zb
+  // (h, m) pieces in Vx, the third pieces here, [2][NM][NM]) and the
+  // walkers' G split likewise for the exchange product ([nw][2NM] each)
+  uint32_t* Vx3 = nullptr;
+  zb* G3 = nullptr;
+  uint32_t* G3l = nullptr;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
   void* vws = nullptr;    // split-k partials of the bias contraction
@@ -134,6 +142,7 @@ struct afqmc_ctx {
   bool ortho_householder = false;  // always use the blocked Householder kernel
   bool fused_prop = false;         // split-BF16 propagator in one kernel when the shapes allow
   bool gemm_adma = true;           // split-BF16 GEMMs stage A by direct global -> LDS loads
+  bool gemm_ring = true;           // ... and both operands through a ring of LDS stages where they allow
   long graph_gen = 0;              // bumped whenever captured substeps become stale
   long exch_chunk = 2048;          // dense exchange: columns per accumulation chunk (0: one pass)
   zd* oU = nullptr;   // [2nw][N][N] R^{-1}
@@ -150,7 +159,7 @@ struct afqmc_ctx {
   // GEMM tile autotuning: call-site key -> candidate index
   std::map<std::string, int> tune;
   bool tuning = false;
-  int force_cfg = 0;  // 1-based tile candidate forced by afqmc_zgemm (tests)
+  int force_cfg = 0;  // 1-based tile candidate forced on every GEMM (AFQMC_OPT_TILE, afqmc_zgemm)
 
   // graphs
   bool use_graphs = false;
@@ -371,18 +380,119 @@ template <int CP> int default_cfg(const ZGemmArgs& a)
   return 0;                                                 // one-wave 32 x 16 tiles
 }
 
+// Split-BF16 products whose operands are both split-BF16 with k contiguous
+// run zgemm_b16dma_kernel (both operands staged by direct loads through a
+// ring of LDS stages); candidates (rows, complex columns, waves, stages):
+struct RingCfg {
+  int BM, BN, WM, WN, S;
+};
+constexpr RingCfg kRing[] = {{128, 128, 2, 2, 2}, {256, 128, 4, 2, 2}, {128, 256, 2, 4, 2}, {256, 128, 4, 2, 3}};
+// three-piece operands take half as much LDS again per stage
+constexpr RingCfg kRing3[] = {{128, 128, 2, 2, 2}, {256, 128, 4, 2, 2}, {128, 256, 2, 4, 2}, {128, 128, 2, 2, 3}};
+constexpr int kNRing = 4;
+template <int NP> constexpr RingCfg ring_at(int i) { return NP == 3 ? kRing3[i] : kRing[i]; }
+
+bool ring_ok(const afqmc_ctx* c, const ZGemmArgs& a)
+{
+  return c->gemm_adma && c->gemm_ring && a.sak == 1 && a.sbk == 1 && a.k % 16 == 0 && a.sam % 2 == 0 &&
+         a.sa1 % 2 == 0 && a.sa2 % 2 == 0 && a.sbn % 2 == 0 && a.sb1 % 2 == 0 && a.sb2 % 2 == 0 &&
+         ((uintptr_t)a.A & 15) == 0 && ((uintptr_t)a.B & 15) == 0 && !a.conjb;
+}
+
+template <class OT, int RI, int NP> int ring_cfg(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
+{
+  constexpr RingCfg t = ring_at<NP>(RI);
+  a.MT = (a.m + t.BM - 1) / t.BM;
+  a.NT = (a.n + t.BN - 1) / t.BN;
+  const long tiles = (long)a.MT * a.NT * batch;
+  if (tiles > 2147483647L) return fail(c, AFQMC_ERR_SHAPE, "GEMM grid too large");
+  return launch(c, name, [&]() {
+    hipLaunchKernelGGL((zgemm_b16dma_kernel<OT, t.BM, t.BN, t.WM, t.WN, t.S, NP>), dim3((unsigned)tiles),
+                       dim3(64 * t.WM * t.WN), 0, c->stream, a);
+  });
+}
+
+template <class OT, int NP> int ring_run(afqmc_ctx* c, const char* name, const ZGemmArgs& a, int batch, int ri)
+{
+  switch (ri) {
+  case 1: return ring_cfg<OT, 1, NP>(c, name, a, batch);
+  case 2: return ring_cfg<OT, 2, NP>(c, name, a, batch);
+  case 3: return ring_cfg<OT, 3, NP>(c, name, a, batch);
+  default: return ring_cfg<OT, 0, NP>(c, name, a, batch);
+  }
+}
+
+int ring_default(const ZGemmArgs& a)
+{
+  if (a.m >= 256 && a.n >= 128) return 1;
+  if (a.m >= 128 && a.n >= 256) return 2;
+  return 0;
+}
+
+template <class OT, int NP = 2> int zgemm_ring(afqmc_ctx* c, const char* name, const ZGemmArgs& a, int batch)
+{
+  char keyb[160];
+  std::snprintf(keyb, sizeof(keyb), "%s/ring%d/%d/%d/%d/%d", name, NP, a.m, a.n, a.k, batch);
+  const std::string key(keyb);
+  if (c->force_cfg > 0 && c->force_cfg <= kNRing) return ring_run<OT, NP>(c, name, a, batch, c->force_cfg - 1);
+  auto it = c->tune.find(key);
+  if (it != c->tune.end() && !c->tuning) return ring_run<OT, NP>(c, name, a, batch, it->second);
+  if (!c->tuning) return ring_run<OT, NP>(c, name, a, batch, ring_default(a));
+  const bool prof = c->prof;
+  c->prof = false;
+  hipEvent_t e0 = get_event(c), e1 = get_event(c);
+  float tms[kNRing];
+  for (int ri = 0; ri < kNRing; ri++) tms[ri] = 1e30f;
+  for (int round = 0; round < 2; round++)
+    for (int ri = 0; ri < kNRing; ri++) {
+      RET((ring_run<OT, NP>(c, name, a, batch, ri)));  // warm
+      (void)hipEventRecord(e0, c->stream);
+      for (int rep = 0; rep < 3; rep++) RET((ring_run<OT, NP>(c, name, a, batch, ri)));
+      (void)hipEventRecord(e1, c->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      tms[ri] = std::min(tms[ri], ms);
+    }
+  int best = ring_default(a);
+  for (int ri = 0; ri < kNRing; ri++)
+    if (tms[ri] < tms[best]) best = ri;
+  c->evpool.push_back(e0);
+  c->evpool.push_back(e1);
+  c->prof = prof;
+  c->tune[key] = best;
+  return ring_run<OT, NP>(c, name, a, batch, best);
+}
+
+// three-piece split-BF16 operands (CP_BF16X6): the staged-ring kernel only
+bool ring3_ok(const ZGemmArgs& a)
+{
+  return a.A3 && a.B3 && a.sak == 1 && a.sbk == 1 && a.k % 16 == 0 && a.sam % 4 == 0 && a.sa1 % 4 == 0 &&
+         a.sa2 % 4 == 0 && a.sbn % 4 == 0 && a.sb1 % 4 == 0 && a.sb2 % 4 == 0 && ((uintptr_t)a.A & 15) == 0 &&
+         ((uintptr_t)a.B & 15) == 0 && ((uintptr_t)a.A3 & 15) == 0 && ((uintptr_t)a.B3 & 15) == 0 && !a.conjb;
+}
+
 template <int CP, class AT, class BT, class OT, bool AK, bool BK>
 int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
 {
   if (a.m <= 0 || a.n <= 0 || batch <= 0) return AFQMC_OK;
   if (a.bdiv <= 0) a.bdiv = 1;
+  if constexpr (CP == CP_BF16X6) {
+    static_assert(std::is_same<AT, bfs>::value && std::is_same<BT, bfs>::value && AK && BK,
+                  "three-piece products take split operands with k contiguous");
+    if (!ring3_ok(a)) return fail(c, AFQMC_ERR_UNSUPPORTED, std::string(name) + ": three-piece operand layout");
+    return zgemm_ring<OT, 3>(c, name, a, batch);
+  } else {
+  if constexpr (CP == CP_BF16X3 && std::is_same<AT, bfs>::value && std::is_same<BT, bfs>::value && AK && BK) {
+    if (ring_ok(c, a)) return zgemm_ring<OT>(c, name, a, batch);
+  }
   char keyb[160];
   std::snprintf(keyb, sizeof(keyb), "%s/%d/%d/%d/%d/%d/%d%d", name, CP, a.m, a.n, a.k, batch, (int)AK, (int)BK);
   const std::string key(keyb);
-  auto it = c->tune.find(key);
-  if (it != c->tune.end() && !c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, it->second);
   if (c->force_cfg > 0 && c->force_cfg <= ncfg<CP>())
     return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, c->force_cfg - 1);
+  auto it = c->tune.find(key);
+  if (it != c->tune.end() && !c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, it->second);
   if (!c->tuning) return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, default_cfg<CP>(a));
   // tuning: time each candidate (results of this call are discarded), in
   // two rounds keeping each candidate's faster round
@@ -411,6 +521,7 @@ int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
   c->prof = prof;
   c->tune[key] = best;
   return zgemm_run<CP, AT, BT, OT, AK, BK>(c, name, a, batch, best);
+  }
 }
 
 ZGemmArgs gargs()
@@ -495,7 +606,16 @@ int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, in
       hipLaunchKernelGGL((gj_inverse_kernel<128, 4>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N,
                          ms, flags, dc);
     });
-  return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 128 electrons per spin is not supported by the batched inverse");
+  if (N <= GJ_GMAX) {
+    // the working matrices do not fit one workgroup's registers: they stay
+    // in global memory (allocated at the first call)
+    if (!c->gjwork) RET(dalloc(c, &c->gjwork, (size_t)batch * ms));
+    return launch(c, "gj_inverse", [&]() {
+      hipLaunchKernelGGL(gj_inverse_global_kernel, dim3(batch), dim3(1024), 0, c->stream, T, Tinv, c->gjwork, det, dq,
+                         dr, N, ms, flags, dc);
+    });
+  }
+  return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 256 electrons per spin is not supported by the batched inverse");
 }
 
 // ... of the context's overlap matrices, with its flags and determinant cache
@@ -1012,7 +1132,10 @@ int do_weights(afqmc_ctx* c, const double* dtp, double dt)
 }
 
 constexpr int kLqNB = 8;
-size_t lq_lds_bytes(int M) { return sizeof(zd) * ((size_t)kLqNB * M + 128 * kLqNB + kLqNB * kLqNB + 128 + 4 * kLqNB * kLqNB); }
+size_t lq_lds_bytes(int M, int N)
+{
+  return sizeof(zd) * ((size_t)kLqNB * M + 128 * kLqNB + kLqNB * kLqNB + std::max(N, 128) + 4 * kLqNB * kLqNB);
+}
 
 // Cholesky QR declines a matrix whose pivot falls below this fraction of
 // the largest diagonal entry of its Gram matrix (condition number above
@@ -1083,7 +1206,7 @@ int do_ortho(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
   c->g_fresh = false;
-  const size_t lds = lq_lds_bytes(c->M);
+  const size_t lds = lq_lds_bytes(c->M, c->N);
   const int M = c->M, N = c->N, nb = 2 * c->nw;
   if (c->ortho_householder || N > 128) {
     return launch(c, "lq_orthogonalize", [&]() {
@@ -1165,7 +1288,15 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv, const void* Lx
         a.k = (int)kc;
         a.ar = inv_dt;
         a.ktri = 1;
-        RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+        if constexpr (CP == CP_BF16X6) {
+          // the product reads the three-piece G; the dot below keeps the FP64 G
+          a.A = c->G3 + sp * NM + q0;
+          a.A3 = c->G3l + sp * NM + q0;
+          a.B3 = c->Vx3 + sp * NM * NM + q0 * NM + q0;
+          RET((zgemm<CP, bfs, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+        } else {
+          RET((zgemm<CP, GT, VXT, YT, true, true>(c, "energy_vx_gemm", a, 1)));
+        }
         RET(launch(c, "energy_exch_dot", [&]() {
           hipLaunchKernelGGL((coldot_kernel<typename ztype<GT>::type, ZT>), dim3(nw), dim3(256), 0, c->stream,
                              (int)(NM - q0), (const typename ztype<GT>::type*)Gp + sp * NM + q0, 1L, 2 * NM,
@@ -1202,7 +1333,11 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv, const void* Lx
     a.n = N;
     a.k = M;
     a.ar = std::sqrt(inv_dt);  // Y of the unscaled vectors
-    RET((zgemm<CP, LT_, GT, YT, true, true>(c, "energy_y_gemm", a, 2 * nw)));
+    if constexpr (CP == CP_BF16X6) {
+      return fail(c, AFQMC_ERR_STATE, "three-piece exchange without its dense block");
+    } else {
+      RET((zgemm<CP, LT_, GT, YT, true, true>(c, "energy_y_gemm", a, 2 * nw)));
+    }
     RET(launch(c, "energy_trace", [&]() {
       hipLaunchKernelGGL((trace_sq_kernel<typename ztype<YT>::type>), dim3(2 * nw), dim3(256), 0, c->stream,
                          (const typename ztype<YT>::type*)c->Ybuf, N, cnt, (long)cnt * per_item, c->e_exch_part, 1L,
@@ -1287,6 +1422,35 @@ template <class ZT> int maybe_vx(afqmc_ctx* c, const zd* Lt64, size_t zsize)
   return build_vx<float>(c, Lt64);
 }
 
+// Vx as three-piece split-BF16 (split-BF16 mode, energy level 2): built in
+// FP32 like the FP32 block, then split in place into its (h, m) pieces with
+// the third pieces alongside; the walkers' G is split likewise per energy.
+int maybe_vx3(afqmc_ctx* c, const zd* Lt64)
+{
+  if (!c->allow_vx) return AFQMC_OK;
+  const long NM = (long)c->N * c->M, nw = c->nw;
+  const size_t bytes = (sizeof(zb) + sizeof(uint32_t)) * 2 * (size_t)NM * NM;
+  const size_t gbytes = (sizeof(zb) + sizeof(uint32_t)) * 2 * (size_t)NM * nw;
+  const size_t ybytes = sizeof(zf) * (size_t)nw * NM;
+  size_t fr = 0, tot = 0;
+  HIPCHK(c, hipMemGetInfo(&fr, &tot));
+  if (bytes + gbytes + ybytes + ((size_t)8 << 30) > fr) return AFQMC_OK;
+  RET(dalloc_bytes(c, &c->Vx, sizeof(zb) * 2 * (size_t)NM * NM));
+  RET(dalloc(c, &c->Vx3, 2 * (size_t)NM * NM));
+  RET(dalloc(c, &c->G3, 2 * (size_t)NM * nw));
+  RET(dalloc(c, &c->G3l, 2 * (size_t)NM * nw));
+  if (c->ybuf_bytes < ybytes) {
+    dfree(c, c->Ybuf);
+    c->ybuf_bytes = ybytes;
+    RET(dalloc_bytes(c, &c->Ybuf, ybytes));
+  }
+  RET(build_vx<float>(c, Lt64));
+  return launch(c, "vx_split3", [&]() {
+    hipLaunchKernelGGL((split3_kernel<zf>), dim3(nblk(2 * NM * NM, 256)), dim3(256), 0, c->stream, (const zf*)c->Vx,
+                       (zb*)c->Vx, c->Vx3, 2 * NM * NM);
+  });
+}
+
 int do_energy(afqmc_ctx* c, double* eav_out)
 {
   const int nw = c->nw, N = c->N, M = c->M;
@@ -1328,7 +1492,15 @@ int do_energy(afqmc_ctx* c, double* eav_out)
     const void *G = c->Gtmp, *L64 = c->Lt64;
     if (c->energy_level >= 3)
       RET((energy_cholesky<CP_F64, double, CP_F64, double, double, double, double>(c, G, L64, L64)));
-    else if (c->energy_level == 2)
+    else if (c->energy_level == 2 && c->Vx3) {
+      // the exchange in three-piece split-BF16 (24-bit operands, FP32
+      // accumulation): G split from its FP64 copy
+      RET(launch(c, "g_split3", [&]() {
+        hipLaunchKernelGGL((split3_kernel<zd>), dim3(nblk(2L * N * M * nw, 256)), dim3(256), 0, c->stream,
+                           (const zd*)c->Gtmp, c->G3, c->G3l, 2L * N * M * nw);
+      }));
+      RET((energy_cholesky<CP_F64, double, CP_BF16X6, double, double, float, bfs>(c, G, L64, L64)));
+    } else if (c->energy_level == 2)
       RET((energy_cholesky<CP_F64, double, CP_F32, double, double, float, float>(c, G, L64, L64)));
     else if (c->bsplit)
       RET((energy_cholesky<CP_F64, double, CP_BF16X3, double, bfs, float, bfs>(c, G, L64, c->Lt)));
@@ -1503,8 +1675,8 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   *out = nullptr;
   if (nmo <= 0 || naea <= 0 || naea > nmo || nchol <= 0 || nwalk <= 0) return AFQMC_ERR_SHAPE;
   if (mode < AFQMC_FP64_CSR || mode > AFQMC_MIXED_BF16) return AFQMC_ERR_ARG;
-  if (naea > 128) return AFQMC_ERR_UNSUPPORTED;
-  if (lq_lds_bytes(nmo) > 160 * 1024) return AFQMC_ERR_UNSUPPORTED;
+  if (naea > GJ_GMAX) return AFQMC_ERR_UNSUPPORTED;
+  if (lq_lds_bytes(nmo, naea) > 160 * 1024) return AFQMC_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return AFQMC_ERR_HIP;
   afqmc_ctx* c = new afqmc_ctx;
@@ -1520,7 +1692,7 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
     return AFQMC_ERR_HIP;
   }
   if (hipFuncSetAttribute((const void*)lq_blocked_kernel<kLqNB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)lq_lds_bytes(nmo)) != hipSuccess) {
+                          (int)lq_lds_bytes(nmo, naea)) != hipSuccess) {
     (void)hipStreamDestroy(c->stream);
     delete c;
     return AFQMC_ERR_HIP;
@@ -1624,7 +1796,14 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     c->fused_prop = value != 0.0;
     return AFQMC_OK;
   }
+  if (option == AFQMC_OPT_TILE) {
+    if (value < 0.0 || value > 8.0) return fail(c, AFQMC_ERR_ARG, "tile candidate out of range");
+    c->force_cfg = (int)value;
+    drop_graphs(c);
+    return AFQMC_OK;
+  }
   if (option == AFQMC_OPT_DIRECT_LDS) {
+    c->gemm_ring = value >= 2.0;
     drop_graphs(c);
     c->gemm_adma = value != 0.0;
     return AFQMC_OK;
@@ -1817,6 +1996,7 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     dfree(c, val);
     if (!c->mixed || c->energy_level >= 3) RET(maybe_vx<zd>(c, Lt64, sizeof(zd)));
     else if (c->bsplit && c->energy_level <= 1) RET(maybe_vx<zb>(c, Lt64, sizeof(zf)));
+    else if (c->bsplit) RET(maybe_vx3(c, Lt64));
     else RET(maybe_vx<zf>(c, Lt64, sizeof(zf)));
     if (c->mixed) {
       zf* Ld32 = nullptr;

@@ -25,7 +25,7 @@
 #include "common.hpp"
 #include <type_traits>
 
-enum { CP_F64 = 0, CP_F32 = 1, CP_BF16X3 = 2 };
+enum { CP_F64 = 0, CP_F32 = 1, CP_BF16X3 = 2, CP_BF16X6 = 3 };
 
 struct ZGemmArgs {
   const void* A;
@@ -56,6 +56,14 @@ struct ZGemmArgs {
   // F(m,n) at F[m*sfm + n*sfn] (complex elements; batch offsets as C's)
   // when sfm or sfn is nonzero, else with C's strides
   long sfm, sfn;
+  // three-piece split-BF16 operands (zgemm_b16dma_kernel with NP = 3): the
+  // third pieces of A and B, one 32-bit word per complex element, with A's
+  // and B's element strides and batch offsets
+  const void* A3;
+  const void* B3;
+  // and of a split-BF16 output C (when not null): its third pieces, with C's
+  // element strides and batch offsets
+  void* C3;
 };
 
 template <int CP> struct LdsT;

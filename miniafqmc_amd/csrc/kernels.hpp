@@ -152,6 +152,145 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
   }
 }
 
+// The same Gauss-Jordan elimination for N too large for the registers of one
+// workgroup (N <= GJ_GMAX): the working matrix X = T^T lives in global memory
+// (work + b*mstride, X(i, j) at [j*N + i], i.e. column j of X contiguous),
+// one workgroup of 1024 threads per matrix; per step the pivot column, the
+// pivot row and the old row k go through LDS and every thread updates its
+// entries of X with the register kernel's arithmetic, so pivots, determinant
+// and inverse follow the same operations.  The result is written to Tinv with
+// the row interchanges undone as column interchanges; T may equal Tinv (work
+// must not).
+constexpr int GJ_GMAX = 256;
+__global__ __launch_bounds__(1024) void gj_inverse_global_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
+                                                                  zd* __restrict__ work, zd* __restrict__ det,
+                                                                  long dq, long dr, int N, long mstride,
+                                                                  int* __restrict__ info, zd* __restrict__ dcache)
+{
+  __shared__ double red[GJ_GMAX];
+  __shared__ zd prow[GJ_GMAX], oldk[GJ_GMAX], colk[GJ_GMAX];
+  __shared__ int perm[GJ_GMAX], dest[GJ_GMAX];
+  __shared__ int piv_row;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long b = blockIdx.x;
+  const zd* Tb = T + b * mstride;
+  zd* X = work + b * mstride;
+  const long NN = (long)N * N;
+  // X(i, j) = T[j][i]: X's column j is T's row j, so the copy is contiguous
+  for (long e = tid; e < NN; e += 1024) X[e] = Tb[e];
+  __syncthreads();
+  zd dt{1.0, 0.0};
+  bool singular = false;
+  for (int k = 0; k < N; k++) {
+    // pivot: first i >= k maximising |Re| + |Im| of X(i, k)
+    for (int i = tid; i < GJ_GMAX; i += 1024) {
+      const zd xk = i < N ? X[(long)k * N + i] : zd{0, 0};
+      if (i < N) colk[i] = xk;
+      red[i] = (i >= k && i < N) ? zcabs1(xk) : -1.0;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double bv = -2.0;
+      int bi = 0;
+      for (int q = lane; q < GJ_GMAX; q += 64) {
+        const double v = red[q];
+        if (v > bv) { bv = v; bi = q; }
+      }
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) {
+        piv_row = bi;
+        perm[k] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = piv_row;
+    for (int j = tid; j < N; j += 1024) {
+      prow[j] = X[(long)j * N + p];
+      if (p != k) oldk[j] = X[(long)j * N + k];
+    }
+    __syncthreads();
+    const zd piv = prow[k];
+    if (piv.re == 0.0 && piv.im == 0.0) {
+      singular = true;
+      dt = zd{0.0, 0.0};
+      break;
+    }
+    dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
+    const zd ipv = zdiv(zd{1.0, 0.0}, piv);
+    for (long e = tid; e < NN; e += 1024) {
+      const int j = (int)(e / N), i = (int)(e % N);
+      zd x;
+      if (i == k) {
+        // new row k = pivot row / piv; column k holds 1/piv
+        x = (j == k) ? ipv : zmul(prow[j], ipv);
+      } else {
+        const bool swapped_in = (i == p);  // p != k here since i != k
+        const zd xo = swapped_in ? oldk[j] : X[e];
+        const zd f = swapped_in ? colk[k] : colk[i];
+        const zd gq = zmul(zd{-f.re, -f.im}, ipv);  // -X(i,k)/piv
+        const zd pr = prow[j];
+        const bool kk = (j == k);
+        const double xr = kk ? 0.0 : xo.re, xi = kk ? 0.0 : xo.im;
+        const double pr_r = kk ? 1.0 : pr.re, pr_i = kk ? 0.0 : pr.im;
+        x.re = xr + gq.re * pr_r - gq.im * pr_i;
+        x.im = xi + gq.re * pr_i + gq.im * pr_r;
+      }
+      X[e] = x;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (singular && info) atomicAdd(info, 1);
+    det[(b / 2) * dq + (b % 2) * dr] = dt;
+    if (dcache) dcache[b] = dt;
+  }
+  if (!Tinv) return;
+  zd* Ob = Tinv + b * mstride;
+  if (singular) {
+    __syncthreads();
+    for (long e = tid; e < NN; e += 1024) Ob[e] = zd{0, 0};
+    return;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int c = 0; c < N; c++) dest[c] = c;  // column map first
+    for (int k = N - 1; k >= 0; k--) {
+      const int pk = perm[k];
+      const int tmp = dest[k];
+      dest[k] = dest[pk];
+      dest[pk] = tmp;
+    }
+    for (int c = 0; c < N; c++) perm[dest[c]] = c;  // inverse map
+  }
+  __syncthreads();
+  // T^-1[perm[j]][i] = X(i, j)
+  for (long e = tid; e < NN; e += 1024) {
+    const int j = (int)(e / N), i = (int)(e % N);
+    Ob[(long)perm[j] * N + i] = X[e];
+  }
+}
+
+// Three-piece split-BF16 of n complex values: (h, m) to hm (a zb per
+// element; may alias the input when IT is zf, each element being read before
+// its slot is written by the same thread) and the third pieces to l.
+template <class IT>
+__global__ void split3_kernel(const IT* in, zb* hm, uint32_t* __restrict__ l, long n)
+{
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const IT v = in[i];
+    zb h;
+    uint32_t t;
+    split3_z((double)v.re, (double)v.im, h, t);
+    hm[i] = h;
+    l[i] = t;
+  }
+}
+
 // W_data[w][2+s] = dcache[2w+s] (determinants cached by the last inverse of
 // the same walkers)
 __global__ void det_restore_kernel(zd* __restrict__ Wd, const zd* __restrict__ dcache, int nw)
@@ -904,10 +1043,10 @@ __global__ __launch_bounds__(256) void lq_blocked_kernel(zd* __restrict__ Wall, 
   if (only && only[blockIdx.x] == 0) return;
   extern __shared__ zd sm[];
   zd* P = sm;                  // [NB][M]
-  zd* W1 = P + NB * M;         // [128][NB]
+  zd* W1 = P + NB * M;         // [128][NB] (trailing rows go in chunks of 128)
   zd* T = W1 + 128 * NB;       // [NB][NB]
-  zd* tau = T + NB * NB;       // [N] (N <= 128)
-  zd* red = tau + 128;         // [4][NB*NB]
+  zd* tau = T + NB * NB;       // [N]
+  zd* red = tau + N;           // [4][NB*NB]
   __shared__ double sh[8];
   zd* W = Wall + (long)blockIdx.x * M * N;
   const int tid = threadIdx.x;

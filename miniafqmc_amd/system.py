@@ -150,21 +150,22 @@ def preset_system(name, **over):
                        p["seed"], p["with_vakbl"]), p
 
 
-# An uneven batch of independent systems (the benchmark definition's
-# multi-system case without its 640-orbital member, whose 160 electrons per
-# spin exceed the batched inverse): M = 96 ... 512, N = M/4, K = 4M,
-# 15-40 % density, 64-512 walkers.  sigma_L keeps M*K*E|L|^2 at the
-# 64-orbital preset's value.
+# The benchmark definition's uneven batch of independent systems:
+# M = 96 ... 640, N = M/4, K = 4M, 15-40 % density, 64-512 walkers.
+# sigma_L keeps M*K*E|L|^2 at the 64-orbital preset's value.
 MULTI = [(96, 64, 0.15), (128, 96, 0.2), (192, 128, 0.25), (256, 160, 0.3), (320, 192, 0.35), (384, 256, 0.4),
-         (512, 512, 0.4)]
+         (512, 512, 0.4), (640, 128, 0.4)]
+
+
+def multi_system(i, dt=0.005):
+    """(System, walkers, seed) of member i of the uneven batch."""
+    M, nw, rho = MULTI[i]
+    N, K = M // 4, 4 * M
+    seed = 20261202 + 5000 + i
+    s = make_system(M, N, K, density=rho, sigma_L=_sigma_for(M, K, rho), sigma_h=1.0, dt=dt, seed=seed)
+    return s, nw, seed
 
 
 def multi_systems(dt=0.005):
     """[(System, walkers, seed)] of the uneven batch."""
-    out = []
-    for i, (M, nw, rho) in enumerate(MULTI):
-        N, K = M // 4, 4 * M
-        seed = 20261202 + 5000 + i
-        s = make_system(M, N, K, density=rho, sigma_L=_sigma_for(M, K, rho), sigma_h=1.0, dt=dt, seed=seed)
-        out.append((s, nw, seed))
-    return out
+    return [multi_system(i, dt) for i in range(len(MULTI))]

@@ -48,7 +48,7 @@ def test_ctx_create_rejects_bad_shapes_without_gpu():
     assert nat.lib.afqmc_ctx_create(0, 4, 8, 2, 0, C.byref(h)) == 2   # AFQMC_ERR_SHAPE
     assert nat.lib.afqmc_ctx_create(8, 9, 8, 2, 0, C.byref(h)) == 2   # N > M
     assert nat.lib.afqmc_ctx_create(8, 4, 8, 2, 7, C.byref(h)) == 1   # bad mode
-    assert nat.lib.afqmc_ctx_create(300, 200, 8, 2, 0, C.byref(h)) == 8  # N > 128 unsupported
+    assert nat.lib.afqmc_ctx_create(600, 300, 8, 2, 0, C.byref(h)) == 8  # N > 256 unsupported
     assert nat.lib.afqmc_ctx_create(8, 4, 8, 2, 0, None) == 1
 
 

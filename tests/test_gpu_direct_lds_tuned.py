@@ -1,6 +1,7 @@
-"""Direct LDS staging with every tile shape the autotuner may pick: the
-tuned split-BF16 walk gives the same walkers as the tuned register-staged
-walk (all tile shapes accumulate k in the same order)."""
+"""Direct LDS staging (a ring of LDS stages for both split operands) with the
+tile shapes the autotuner picks: the tuned split-BF16 walk gives the same
+walkers as the tuned register-staged walk (all tile shapes accumulate k in
+the same order)."""
 import sys
 
 import numpy as np
@@ -22,8 +23,8 @@ def test_direct_lds_tuned_tiles_match_register_path(require_gpu, shape):
     nw = 128
     x = m.fields_block(p["seed"], s.nchol, nw, 0, 3)
     out = []
-    for direct in (False, True):
-        say(f"tuned direct LDS {shape} on={direct}")
+    for direct in (0, 2):
+        say(f"tuned direct LDS {shape} level={direct}")
         g = m.AfqmcSys(s, nw, "mixed_bf16", dense_exchange=False)
         g.set_direct_lds(direct)
         g.stage_fields(x)

@@ -526,8 +526,9 @@ def test_fused_propagator_matches_gemm_chain(require_gpu, shape):
 
 @pytest.mark.parametrize("shape", [(40, 8, 64), (48, 12, 96), "N64_M256", "N128_M512"])
 def test_direct_lds_staging_matches_register_path(require_gpu, shape):
-    """Split-BF16 GEMMs whose left operand goes from global memory straight
-    into LDS (swizzled tile) give the walkers, overlaps and energies of the
+    """Split-BF16 GEMMs whose operands go from global memory straight into
+    LDS (swizzled tiles; level 1 two stages for the left operand, level 2 a
+    ring of stages for both) give the walkers, overlaps and energies of the
     register-staged path bit for bit; 40 and 48 orbitals leave partial row
     tiles (rows past m read a clamped row whose outputs are dropped), and
     at 40 orbitals the propagator's k = 40 is not a multiple of 16, so only
@@ -542,8 +543,8 @@ def test_direct_lds_staging_matches_register_path(require_gpu, shape):
     nw = 16
     x = m.fields_block(seed, s.nchol, nw, 0, 4)
     out = []
-    for direct in (False, True):
-        say(f"direct LDS staging {shape} on={direct}")
+    for direct in (0, 1, 2):
+        say(f"direct LDS staging {shape} level={direct}")
         g = m.AfqmcSys(s, nw, "mixed_bf16", dense_exchange=False)
         g.set_direct_lds(direct)
         g.init_walkers()
@@ -553,5 +554,34 @@ def test_direct_lds_staging_matches_register_path(require_gpu, shape):
         g.run(1, 4, 2, s.dt)
         out.append((g.W, g.W_data))
         g.close()
+    for lv in (1, 2):
+        for a, b in zip(out[0], out[lv]):
+            assert np.array_equal(a, b), f"level {lv}: max diff {np.abs(a - b).max():.3e}"
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
+def test_every_ring_tile_matches_register_path(require_gpu, tile):
+    """Every tile shape of the staged-ring split-BF16 GEMM (forced on all call
+    sites) gives the walkers, overlaps and local energies of the
+    register-staged path bit for bit at 512 orbitals / 128 electrons (the
+    Taylor steps, P U, vHS, the bias product and the triangular exchange
+    block in split-BF16 at energy level 0, exchange chunks of 2048)."""
+    s = m.make_system(512, 128, 256, density=1.0, sigma_L=0.02 * (64.0 * 256 / (512 * 256)) ** 0.5, seed=31)
+    nw = 24
+    x = m.fields_block(31, s.nchol, nw, 0, 3)
+    out = []
+    for direct in (0, 2):
+        say(f"ring tile {tile}, level {direct}")
+        g = m.AfqmcSys(s, nw, "mixed_bf16", energy_precision=0)
+        g.set_exchange_chunk(2048)
+        g.set_direct_lds(direct)
+        g.set_tile(tile)
+        g.init_walkers()
+        g.stage_fields(x)
+        g.calculate_mixed_density_matrix()
+        g.calculate_energy()
+        g.run(1, 3, 2, s.dt)
+        out.append((g.W, g.W_data))
+        g.close()
     for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b), f"max diff {np.abs(a - b).max():.3e}"
+        assert np.array_equal(a, b), f"tile {tile}: max diff {np.abs(a - b).max():.3e}"
