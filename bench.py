@@ -230,6 +230,9 @@ def main():
     ap.add_argument("--direct-lds", type=int, default=2,
                     help="split-BF16 GEMMs: 2 both split operands through a ring of LDS stages by direct loads, "
                          "1 the left operand by direct loads into two stages, 0 through registers")
+    ap.add_argument("--three-piece", type=int, default=0,
+                    help="split-BF16 mode: contractions in three-piece split-BF16 (24-bit operands), a mask: "
+                         "1 bias product, 2 HS potential, 4 propagator")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
                          "hardware byte counters)")
@@ -299,7 +302,8 @@ def main():
     def run_mode(mode, profile, state=None):
         """Time `args.steps` substeps in `mode`; then run the accuracy probes
         from `state`, or from this run's own final walkers (returned)."""
-        g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level)
+        g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level,
+                       three_piece=args.three_piece if mode == "mixed_bf16" else 0)
         g.set_exchange_chunk(args.exchange_chunk)
         g.set_direct_lds(args.direct_lds)
         # the auxiliary fields are drawn on the device inside each substep
@@ -496,6 +500,7 @@ def main():
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
                    "exchange_chunk": args.exchange_chunk if args.energy_level < 3 else None,
                    "graphs": bool(args.graphs), "direct_lds": args.direct_lds,
+                   "three_piece": args.three_piece if args.mode == "mixed_bf16" else 0,
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -135,7 +135,15 @@ typedef enum {
    * (afqmc_tune) or the size heuristic picked; k > 0 forces candidate k of
    * the call site's kind on every GEMM (tests cover every shape this way;
    * results are the same bit for bit).  No counterpart in the reference. */
-  AFQMC_OPT_TILE = 7
+  AFQMC_OPT_TILE = 7,
+  /* AFQMC_MIXED_BF16, set before afqmc_upload_system: a mask of the
+   * contractions run in three-piece split-BF16 (each operand x = h + m + l,
+   * three bf16 pieces, 24 significant bits, six products accumulated in
+   * FP32) instead of two pieces: 1 the bias product (G from an FP32 product),
+   * 2 the HS potential, 4 the propagator (P W, the Taylor terms, P U).
+   * Needs M, K and N*M multiples of 16.  Default 0.  No counterpart in the
+   * reference. */
+  AFQMC_OPT_THREE_PIECE = 8
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

@@ -72,7 +72,7 @@ __device__ inline zb split_z(float re, float im)
 // of x; l goes to a separate array of one word (bf16(re) | bf16(im) << 16)
 // per complex, so two-piece consumers read the same zb array.
 __device__ inline unsigned short d2bf(double x) { return f2bf((float)x); }
-__device__ inline void split3_z(double re, double im, zb& hm, uint32_t& l)
+__device__ __forceinline__ void split3_z(double re, double im, zb& hm, uint32_t& l)
 {
   const unsigned short hr = d2bf(re), hi = d2bf(im);
   const double rr = re - (double)bf2f(hr), ri = im - (double)bf2f(hi);
@@ -80,6 +80,16 @@ __device__ inline void split3_z(double re, double im, zb& hm, uint32_t& l)
   const unsigned short lr = d2bf(rr - (double)bf2f(mr)), li = d2bf(ri - (double)bf2f(mi));
   hm = zb{(uint32_t)hr | ((uint32_t)hi << 16), (uint32_t)mr | ((uint32_t)mi << 16)};
   l = (uint32_t)lr | ((uint32_t)li << 16);
+}
+// the same for an FP32 value, in FP32 arithmetic (x - h and x - h - m are
+// exact in FP32), with (h, m) equal to split_z's
+__device__ __forceinline__ void split3_f(float re, float im, zb& hm, uint32_t& l)
+{
+  const unsigned short hr = f2bf(re), hi = f2bf(im);
+  const float rr = re - bf2f(hr), ri = im - bf2f(hi);
+  const unsigned short mr = f2bf(rr), mi = f2bf(ri);
+  hm = zb{(uint32_t)hr | ((uint32_t)hi << 16), (uint32_t)mr | ((uint32_t)mi << 16)};
+  l = (uint32_t)f2bf(rr - bf2f(mr)) | ((uint32_t)f2bf(ri - bf2f(mi)) << 16);
 }
 __device__ inline zd to_zd(zd v) { return v; }
 __device__ inline zd to_zd(zf v) { return zd{(double)v.re, (double)v.im}; }

@@ -134,6 +134,18 @@ struct afqmc_ctx {
   uint32_t* Vx3 = nullptr;
   zb* G3 = nullptr;
   uint32_t* G3l = nullptr;
+  // three-piece split-BF16 contractions (AFQMC_OPT_THREE_PIECE, split-BF16
+  // mode): bit 1 the bias product, bit 2 the HS potential, bit 4 the
+  // propagator; the third pieces of their operands, beside the split arrays
+  int three = 0;
+  uint32_t* tp_Lt = nullptr;   // [2][K][NM] (bias)
+  uint32_t* tp_G = nullptr;    // [nw][2NM] (bias)
+  uint32_t* tp_Ld = nullptr;   // [M*M][K] (HS potential)
+  uint32_t* tp_X = nullptr;    // [nw][K] (HS potential)
+  uint32_t* tp_P = nullptr;    // [M][M] (propagator)
+  uint32_t* tp_vHS = nullptr;  // [nw][M*M] (propagator)
+  uint32_t* tp_Ta = nullptr;   // [nw][2][N][M] (propagator terms)
+  uint32_t* tp_Tb = nullptr;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
   void* vws = nullptr;    // split-k partials of the bias contraction
@@ -732,6 +744,11 @@ int do_mdm(afqmc_ctx* c)
   // mixed modes: G only feeds the force bias, which contracts it in the
   // contraction precision, so it is formed in that precision as well (the
   // energy rebuilds an FP64 G from the FP64 inverses)
+  if (c->bsplit && (c->three & 1)) {
+    // three-piece bias product: G from an FP32 product, split in three
+    a.C3 = c->tp_G;
+    return zgemm<CP_F32, double, double, bfs, true, true>(c, "gc_gemm", a, 2 * nw);
+  }
   if (c->bsplit) return zgemm<CP_BF16X3, double, double, bfs, true, true>(c, "gc_gemm", a, 2 * nw);
   if (c->mixed) return zgemm<CP_F32, double, double, float, true, true>(c, "gc_gemm", a, 2 * nw);
   return zgemm<CP_F64, double, double, double, true, true>(c, "gc_gemm", a, 2 * nw);
@@ -753,7 +770,8 @@ int do_overlaps(afqmc_ctx* c)
 // the 2S partial products (spin, chunk) run as one batched GEMM into a
 // workspace and are then summed in FP64.
 template <int CP, class GT, class LT_>
-int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, const char* name)
+int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, const char* name,
+                         const uint32_t* G3 = nullptr, const uint32_t* Lt3 = nullptr)
 {
   const long NM = (long)c->N * c->M, nw = c->nw, K = c->K;
   const long tiles = ((nw + 127) / 128) * ((K + 63) / 64);
@@ -775,6 +793,10 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, 
       a.n = (int)K;
       a.k = (int)NM;
       a.br = sp == 0 ? 0.0 : 1.0;
+      if (G3) {
+        a.A3 = G3 + sp * NM;
+        a.B3 = Lt3 + sp * K * NM;
+      }
       RET((zgemm<CP, GT, LT_, double, true, true>(c, name, a, 1)));
     }
     return AFQMC_OK;
@@ -809,6 +831,8 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, 
   a.m = (int)nw;
   a.n = (int)K;
   a.k = (int)chunk;
+  a.A3 = G3;  // third pieces with the same offsets
+  a.B3 = Lt3;
   RET((zgemm<CP, GT, LT_, PT, true, true>(c, name, a, 2 * S)));
   return launch(c, "vbias_reduce", [&]() {
     hipLaunchKernelGGL((zsum_slices_kernel<PZ>), dim3(nblk(nw * K, 256)), dim3(256), 0, c->stream, (const PZ*)c->vws, out,
@@ -874,6 +898,8 @@ int do_vbias(afqmc_ctx* c)
     return dense_contract_vbias<CP_F64, double, double>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
   if (c->mode == AFQMC_MIXED_FP32)
     return dense_contract_vbias<CP_F32, float, float>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
+  if (c->three & 1)
+    return dense_contract_vbias<CP_BF16X6, bfs, bfs>(c, c->G, c->Lt, c->vbias, "vbias_gemm", c->tp_G, c->tp_Lt);
   return dense_contract_vbias<CP_BF16X3, bfs, bfs>(c, c->G, c->Lt, c->vbias, "vbias_gemm");
 }
 
@@ -892,13 +918,13 @@ int do_fields(afqmc_ctx* c, const double* xdev, long substep, const long* xslice
   return launch(c, "fields", [&]() {
     if (!c->mixed)
       hipLaunchKernelGGL((fields_wfirst_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
-                         (zd*)c->X, c->hybridW, c->flags + 1);
+                         (zd*)c->X, c->hybridW, c->flags + 1, (uint32_t*)nullptr);
     else if (c->bsplit)
       hipLaunchKernelGGL((fields_wfirst_kernel<zb>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
-                         (zb*)c->X, c->hybridW, c->flags + 1);
+                         (zb*)c->X, c->hybridW, c->flags + 1, c->tp_X);
     else
       hipLaunchKernelGGL((fields_wfirst_kernel<zf>), dim3(nw), dim3(256), 0, c->stream, K, nw, c->vbias, src,
-                         (zf*)c->X, c->hybridW, c->flags + 1);
+                         (zf*)c->X, c->hybridW, c->flags + 1, (uint32_t*)nullptr);
   });
 }
 
@@ -946,6 +972,12 @@ int do_vhs(afqmc_ctx* c)
   // split-BF16: Ld stored [M*M][K]
   a.sbn = K;
   a.sbk = 1;
+  a.C3 = c->tp_vHS;  // three-piece propagator: vHS with its third pieces
+  if (c->three & 2) {
+    a.A3 = c->tp_X;
+    a.B3 = c->tp_Ld;
+    return zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "vhs_gemm", a, 1);
+  }
   return zgemm<CP_BF16X3, bfs, bfs, bfs, true, true>(c, "vhs_gemm", a, 1);
 }
 
@@ -1030,6 +1062,94 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
   return AFQMC_OK;
 }
 
+// The propagator in three-piece split-BF16: W and S are transposed into
+// three-piece terms ([N][M] per walker and spin) so that every product reads
+// both operands k-contiguous through the staged-ring kernel; the Horner terms
+// are written with their third pieces.
+int propagate_three(afqmc_ctx* c)
+{
+  const int M = c->M, N = c->N, nw = c->nw;
+  const long MN = (long)M * N, MM = (long)M * M;
+  const dim3 tgrid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32), (unsigned)(2 * nw));
+  zb* Ta = (zb*)c->Ta;
+  zb* Tb = (zb*)c->Tb;
+  RET(launch(c, "prop_split_w", [&]() {
+    hipLaunchKernelGGL(transpose_split3_kernel<zd>, tgrid, dim3(256), 0, c->stream, (const zd*)c->W, Tb, c->tp_Tb, M, N);
+  }));
+  auto base = [&]() {
+    ZGemmArgs a = gargs();
+    a.A = c->propg32;
+    a.A3 = c->tp_P;
+    a.sam = M;
+    a.sak = 1;
+    a.sbn = M;
+    a.sbk = 1;
+    a.sb1 = 2 * MN;
+    a.sb2 = MN;
+    a.bdiv = 2;
+    a.m = M;
+    a.n = N;
+    a.k = M;
+    return a;
+  };
+  {  // S = P W
+    ZGemmArgs a = base();
+    a.B = Tb;
+    a.B3 = c->tp_Tb;
+    a.C = c->S;
+    a.scm = N;
+    a.scn = 1;
+    a.sc1 = 2 * MN;
+    a.sc2 = MN;
+    RET((zgemm<CP_BF16X6, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
+  }
+  RET(launch(c, "prop_split_s", [&]() {
+    hipLaunchKernelGGL(transpose_split3_kernel<zf>, tgrid, dim3(256), 0, c->stream, (const zf*)c->S, Tb, c->tp_Tb, M,
+                       N);
+  }));
+  // Horner's rule as propagate_chain: U_7 = S, U_k = S + (i/k) V U_{k+1}
+  zb* prev = Tb;
+  uint32_t* prev3 = c->tp_Tb;
+  zb* bufs[2] = {Ta, Tb};
+  uint32_t* bufs3[2] = {c->tp_Ta, c->tp_Tb};
+  for (int k = 6; k >= 1; k--) {
+    ZGemmArgs a = base();
+    a.A = c->vHS;
+    a.A3 = c->tp_vHS;
+    a.sa1 = MM;
+    a.sa2 = 0;
+    a.B = prev;
+    a.B3 = prev3;
+    a.C = bufs[k & 1];
+    a.C3 = bufs3[k & 1];
+    a.scm = 1;
+    a.scn = M;
+    a.sc1 = 2 * MN;
+    a.sc2 = MN;
+    a.ar = 0.0;
+    a.ai = 1.0 / (double)k;
+    a.F = c->S;
+    a.sfm = N;
+    a.sfn = 1;
+    a.cf = 1;
+    RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
+    prev = bufs[k & 1];
+    prev3 = bufs3[k & 1];
+  }
+  {  // W = P U_1
+    ZGemmArgs a = base();
+    a.B = prev;
+    a.B3 = prev3;
+    a.C = c->W;
+    a.scm = N;
+    a.scn = 1;
+    a.sc1 = 2 * MN;
+    a.sc2 = MN;
+    RET((zgemm<CP_BF16X6, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
+  }
+  return AFQMC_OK;
+}
+
 // W <- P E6(V) P W in one kernel per (walker, 32 columns) (propagate.hpp):
 // split-BF16 mode, M a multiple of 64 up to 512, N a multiple of 16.
 bool fused_prop_ok(const afqmc_ctx* c)
@@ -1063,6 +1183,7 @@ int do_propagate(afqmc_ctx* c)
 {
   c->t1inv_valid = false;
   c->g_fresh = false;
+  if (c->three & 4) return propagate_three(c);
   if (fused_prop_ok(c)) return propagate_fused(c);
   if (c->mode == AFQMC_FP64_CSR) return propagate_chain<CP_F64, double, double, double, double>(c, c->propg, c->vHSw);
   if (c->mode == AFQMC_FP64_DENSE)
@@ -1589,6 +1710,13 @@ inline void host_split(double re, double im, uint32_t w[2])
   w[0] = (uint32_t)hr | ((uint32_t)hi << 16);
   w[1] = (uint32_t)lr | ((uint32_t)li << 16);
 }
+// the third piece of (re, im) given its two-piece split w
+inline uint32_t host_third(double re, double im, const uint32_t w[2])
+{
+  const double rr = re - host_bf2f((uint16_t)(w[0] & 0xffff)) - host_bf2f((uint16_t)(w[1] & 0xffff));
+  const double ri = im - host_bf2f((uint16_t)(w[0] >> 16)) - host_bf2f((uint16_t)(w[1] >> 16));
+  return (uint32_t)host_f2bf((float)rr) | ((uint32_t)host_f2bf((float)ri) << 16);
+}
 inline zd host_merge(const uint32_t w[2])
 {
   return zd{(double)host_bf2f((uint16_t)(w[0] & 0xffff)) + (double)host_bf2f((uint16_t)(w[1] & 0xffff)),
@@ -1796,6 +1924,14 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     c->fused_prop = value != 0.0;
     return AFQMC_OK;
   }
+  if (option == AFQMC_OPT_THREE_PIECE) {
+    if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
+    if (value < 0 || value > 7 || value != (int)value) return fail(c, AFQMC_ERR_ARG, "three-piece mask is 0..7");
+    if (value != 0 && c->mode != AFQMC_MIXED_BF16)
+      return fail(c, AFQMC_ERR_UNSUPPORTED, "three-piece contractions are a split-BF16 option");
+    c->three = (int)value;
+    return AFQMC_OK;
+  }
   if (option == AFQMC_OPT_TILE) {
     if (value < 0.0 || value > 8.0) return fail(c, AFQMC_ERR_ARG, "tile candidate out of range");
     c->force_cfg = (int)value;
@@ -1855,6 +1991,22 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
         hipLaunchKernelGGL((zconvert_kernel<zd, zf>), dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg,
                            (zf*)c->propg32, MM);
     }));
+    if (c->three) {
+      // three-piece contractions need 16-complex k-tiles of aligned rows
+      if (M % 16 || K % 16 || NM % 16) return fail(c, AFQMC_ERR_UNSUPPORTED, "three-piece contractions need M, K and N*M multiples of 16");
+      if (c->three & 4) {
+        RET(dalloc(c, &c->tp_P, MM));
+        RET(launch(c, "third_pieces", [&]() {
+          hipLaunchKernelGGL(third_piece_kernel, dim3(nblk(MM, 256)), dim3(256), 0, st, c->propg, (const zb*)c->propg32,
+                             c->tp_P, MM, 0L, 0L, 0);
+        }));
+        RET(dalloc(c, &c->tp_vHS, (size_t)c->nw * MM));
+        RET(dalloc(c, &c->tp_Ta, (size_t)c->nw * 2 * MN));
+        RET(dalloc(c, &c->tp_Tb, (size_t)c->nw * 2 * MN));
+      }
+      if (c->three & 2) RET(dalloc(c, &c->tp_X, (size_t)c->nw * K));
+      if (c->three & 1) RET(dalloc(c, &c->tp_G, (size_t)c->nw * 2 * NM));
+    }
   }
   // Spvn CSR to the device
   const long nnz = spvn->nnz;
@@ -1883,14 +2035,74 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
         c->sp_sorted = false;
         break;
       }
+  // Repeated (row, column) entries: the reference keeps them (SparseMatrix::
+  // compress only sorts, SparseMatrix.hpp:295-319) and csrmm sums them, so
+  // the CSR products use the matrix as given; the dense forms take each
+  // repeated entry's sum, formed here on the host in entry order.
+  const int* drp = rp;
+  const int* dcol = col;
+  const zd* dval = val;
+  {
+    std::vector<long> last(K, -1), pos(K, 0);
+    bool dup = false;
+    for (long r = 0; r < MM && !dup; r++)
+      for (long q = spvn->rowptr[r] - spvn->rowptr[0]; q < spvn->rowptr[r + 1] - spvn->rowptr[0]; q++) {
+        if (last[scol[q]] == r) {
+          dup = true;
+          break;
+        }
+        last[scol[q]] = r;
+      }
+    if (dup) {
+      std::vector<int> mrp(MM + 1), mcol;
+      std::vector<zd> mval;
+      mcol.reserve(nnz);
+      mval.reserve(nnz);
+      std::fill(last.begin(), last.end(), -1);
+      const zd* sv = (const zd*)sval;
+      for (long r = 0; r < MM; r++) {
+        mrp[r] = (int)mcol.size();
+        for (long q = spvn->rowptr[r] - spvn->rowptr[0]; q < spvn->rowptr[r + 1] - spvn->rowptr[0]; q++) {
+          const int cq = scol[q];
+          if (last[cq] == r) {
+            mval[pos[cq]] = zadd(mval[pos[cq]], sv[q]);
+          } else {
+            last[cq] = r;
+            pos[cq] = (long)mcol.size();
+            mcol.push_back(cq);
+            mval.push_back(sv[q]);
+          }
+        }
+      }
+      mrp[MM] = (int)mcol.size();
+      int *mrp_d = nullptr, *mcol_d = nullptr;
+      zd* mval_d = nullptr;
+      RET(dalloc(c, &mrp_d, MM + 1));
+      RET(dalloc(c, &mcol_d, (long)mcol.size()));
+      RET(dalloc(c, &mval_d, (long)mval.size()));
+      HIPCHK(c, hipMemcpyAsync(mrp_d, mrp.data(), sizeof(int) * (MM + 1), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(mcol_d, mcol.data(), sizeof(int) * mcol.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(mval_d, mval.data(), sizeof(zd) * mval.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      drp = mrp_d;
+      dcol = mcol_d;
+      dval = mval_d;
+    }
+  }
   // dense Ld[n][ik] (FP64), then the half rotation Lt[s][n][a][k] = sum_i trial_s(i,a) Ld[n][i][k]
   zd* Ld64 = nullptr;
   RET(dalloc(c, &Ld64, K * MM));
   HIPCHK(c, hipMemsetAsync(Ld64, 0, sizeof(zd) * K * MM, st));
   RET(launch(c, "csr_densify", [&]() {
-    hipLaunchKernelGGL(csr_densify_kernel, dim3((unsigned)((MM + 3) / 4)), dim3(256), 0, st, (int)MM, rp, col, val,
+    hipLaunchKernelGGL(csr_densify_kernel, dim3((unsigned)((MM + 3) / 4)), dim3(256), 0, st, (int)MM, drp, dcol, dval,
                        Ld64, MM);
   }));
+  if (drp != rp) {
+    HIPCHK(c, hipStreamSynchronize(st));
+    dfree(c, (void*)drp);
+    dfree(c, (void*)dcol);
+    dfree(c, (void*)dval);
+  }
   zd* Lt64 = nullptr;
   RET(dalloc(c, &Lt64, 2 * K * NM));
   for (int sp = 0; sp < 2; sp++) {
@@ -2017,6 +2229,21 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
                              2 * K * NM);
         }
       }));
+      if (c->three & 1) {
+        RET(dalloc(c, &c->tp_Lt, 2 * K * NM));
+        RET(launch(c, "third_pieces", [&]() {
+          hipLaunchKernelGGL(third_piece_kernel, dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64, (const zb*)Lt32,
+                             c->tp_Lt, 2 * K * NM, 0L, 0L, 0);
+        }));
+      }
+      if (c->three & 2) {
+        // split Ld is [M*M][K], the transpose of the FP64 [K][M*M]
+        RET(dalloc(c, &c->tp_Ld, K * MM));
+        RET(launch(c, "third_pieces", [&]() {
+          hipLaunchKernelGGL(third_piece_kernel, dim3(nblk(K * MM, 256)), dim3(256), 0, st, Ld64, (const zb*)Ld32,
+                             c->tp_Ld, K * MM, K, MM, 1);
+        }));
+      }
       HIPCHK(c, hipStreamSynchronize(st));
       dfree(c, Ld64);
       if (c->energy_level >= 1) {
@@ -2076,10 +2303,12 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
     HIPCHK(c, hipStreamSynchronize(st));
     return AFQMC_OK;
   };
-  // walker-last reference [R][nw] <-> walker-first device [nw][R] of type zd/zf/zb
-  auto wlast = [&](void* dev, long R, bool is32) -> int {
+  // walker-last reference [R][nw] <-> walker-first device [nw][R] of type
+  // zd/zf/zb, with the third pieces in dev3 when the buffer has them
+  auto wlast = [&](void* dev, long R, bool is32, uint32_t* dev3 = nullptr) -> int {
     std::vector<zd> tmp((size_t)R * nw);
     std::vector<zf> t32;
+    std::vector<uint32_t> t3(dev3 && c->bsplit ? tmp.size() : 0);
     if (to_device) {
       htranspose((const zd*)host, tmp.data(), R, nw);
       if (is32) {
@@ -2089,11 +2318,14 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
             uint32_t w[2];
             host_split(tmp[i].re, tmp[i].im, w);
             std::memcpy(&t32[i], w, sizeof(w));
+            if (!t3.empty()) t3[i] = host_third(tmp[i].re, tmp[i].im, w);
           } else {
             t32[i] = zf{(float)tmp[i].re, (float)tmp[i].im};
           }
         }
         HIPCHK(c, hipMemcpyAsync(dev, t32.data(), sizeof(zf) * t32.size(), hipMemcpyHostToDevice, st));
+        if (!t3.empty())
+          HIPCHK(c, hipMemcpyAsync(dev3, t3.data(), sizeof(uint32_t) * t3.size(), hipMemcpyHostToDevice, st));
       } else {
         HIPCHK(c, hipMemcpyAsync(dev, tmp.data(), sizeof(zd) * tmp.size(), hipMemcpyHostToDevice, st));
       }
@@ -2102,12 +2334,18 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
       if (is32) {
         t32.resize(tmp.size());
         HIPCHK(c, hipMemcpyAsync(t32.data(), dev, sizeof(zf) * t32.size(), hipMemcpyDeviceToHost, st));
+        if (!t3.empty())
+          HIPCHK(c, hipMemcpyAsync(t3.data(), dev3, sizeof(uint32_t) * t3.size(), hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
         for (size_t i = 0; i < tmp.size(); i++) {
           if (c->bsplit) {
             uint32_t w[2];
             std::memcpy(w, &t32[i], sizeof(w));
             tmp[i] = host_merge(w);
+            if (!t3.empty()) {
+              tmp[i].re += (double)host_bf2f((uint16_t)(t3[i] & 0xffff));
+              tmp[i].im += (double)host_bf2f((uint16_t)(t3[i] >> 16));
+            }
           } else {
             tmp[i] = zd{t32[i].re, t32[i].im};
           }
@@ -2127,11 +2365,11 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
   case AFQMC_BUF_W: return raw(c->W, sizeof(zd) * nw * 2 * MN);
   case AFQMC_BUF_WDATA: return raw(c->Wd, sizeof(zd) * nw * 8);
   case AFQMC_BUF_HYBRIDW: return raw(c->hybridW, sizeof(zd) * nw);
-  case AFQMC_BUF_G: return d ? wlast(c->G, NAK, c->mixed) : raw(c->G, sizeof(zd) * NAK * nw);
+  case AFQMC_BUF_G: return d ? wlast(c->G, NAK, c->mixed, c->tp_G) : raw(c->G, sizeof(zd) * NAK * nw);
   case AFQMC_BUF_VBIAS: return d ? wlast(c->vbias, K, false) : raw(c->vbias, sizeof(zd) * K * nw);
-  case AFQMC_BUF_X: return d ? wlast(c->X, K, c->mixed) : raw(c->X, sizeof(zd) * K * nw);
+  case AFQMC_BUF_X: return d ? wlast(c->X, K, c->mixed, c->tp_X) : raw(c->X, sizeof(zd) * K * nw);
   case AFQMC_BUF_VHS:
-    if (d) return wlast(c->vHS, MM, c->mixed);
+    if (d) return wlast(c->vHS, MM, c->mixed, c->tp_vHS);
     RET(raw(c->vHS, sizeof(zd) * MM * nw));
     if (to_device) {
       dim3 grid((nw + 31) / 32, (unsigned)((MM + 31) / 32));

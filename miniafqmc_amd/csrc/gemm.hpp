@@ -549,8 +549,19 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_kernel(ZGemmArgs p)
           const long off = 2 * ((long)row * p.scm + (long)col * p.scn);
           if (D) D[off + s] = (DT)((double)fv[j][q] + o);
           if constexpr (OSPLIT) {
-            // lane s=0 writes the whole split complex (no beta)
-            if (CS && s == 0) CS[off / 2] = split_z((float)ofc, (float)oi);
+            // lane s=0 writes the whole split complex (no beta), and its
+            // third piece when C3 is set
+            if (CS && s == 0) {
+              if (p.C3) {
+                zb h;
+                uint32_t t;
+                split3_f((float)ofc, (float)oi, h, t);
+                CS[off / 2] = h;
+                ((uint32_t*)p.C3)[coff / 2 + off / 2] = t;
+              } else {
+                CS[off / 2] = split_z((float)ofc, (float)oi);
+              }
+            }
           } else if (C) {
             C[off + s] = (CT)(p.cf ? oc + (double)fv[j][q] : oc);
           }

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
     pb3[i] = B3 + (long)min(n0 + row, p.n - 1) * p.sbn + 4 * (pc ^ b16::swz3(row));
   }
   const int w0 = __builtin_amdgcn_readfirstlane(wave);
-  auto issue = [&](uint2* st, int k0) {
+  auto issue = [&](uint2* st, int k0) __attribute__((always_inline)) {
 #pragma unroll
     for (int i = 0; i < AL; i++)
       __builtin_amdgcn_global_load_lds((const void*)(pa[i] + k0),
@@ -180,11 +180,15 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
   const int c3 = g ^ b16::swz3(r);  // third-piece chunk of this lane's four k
   // fragments of one k-tile: A rows (hi and lo words of four complex k) and
   // the raw B columns, read from LDS before the next loads are issued
+  // (three pieces: the B columns are read per column block inside the MFMA
+  // loop instead, which keeps the register count of eight-wave tiles within
+  // the budget)
+  constexpr int TNF = NP == 3 ? 1 : TN;
   struct Frags {
-    uint4 ah[TM], al[TM], b0[TN], b1[TN];
-    uint4 a3[NP == 3 ? TM : 1], b3[NP == 3 ? TN : 1];
+    uint4 ah[TM], al[TM], b0[TNF], b1[TNF];
+    uint4 a3[NP == 3 ? TM : 1];
   };
-  auto read = [&](const uint2* st, Frags& f) {
+  auto read = [&](const uint2* st, Frags& f) __attribute__((always_inline)) {
     const uint4* t4 = (const uint4*)st;  // 8 uint4 per tile row
 #pragma unroll
     for (int i = 0; i < TM; i++) {
@@ -192,55 +196,68 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
       f.ah[i] = make_uint4(v0.x, v0.z, v1.x, v1.z);
       f.al[i] = make_uint4(v0.y, v0.w, v1.y, v1.w);
     }
+    if constexpr (NP == 2) {
 #pragma unroll
-    for (int j = 0; j < TN; j++) {
-      f.b0[j] = t4[(brow + 16 * j) * 8 + c0];
-      f.b1[j] = t4[(brow + 16 * j) * 8 + c1];
-    }
-    if constexpr (NP == 3) {
+      for (int j = 0; j < TN; j++) {
+        f.b0[j] = t4[(brow + 16 * j) * 8 + c0];
+        f.b1[j] = t4[(brow + 16 * j) * 8 + c1];
+      }
+    } else {
       const uint4* t3 = (const uint4*)(st + (BM + BN) * KT);  // 4 uint4 per third-piece row
 #pragma unroll
       for (int i = 0; i < TM; i++) f.a3[i] = t3[(arow + 16 * i) * 4 + c3];
-#pragma unroll
-      for (int j = 0; j < TN; j++) f.b3[j] = t3[(brow + 16 * j) * 4 + c3];
     }
   };
-  auto mfmas = [&](const Frags& f) {
+  auto mfmas = [&](const Frags& f, const uint2* st) __attribute__((always_inline)) {
+    // real part: (b_re, -b_im); imaginary part: (b_im, b_re)
+    auto ng = [](uint32_t w) { return w ^ 0x80000000u; };
+    auto rt = [](uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16u); };
+    auto mm = [&](int i, int j, const uint4& a, const uint4& br, const uint4& bi) __attribute__((always_inline)) {
+      const bf16x8 xa = __builtin_bit_cast(bf16x8, a);
+      cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, __builtin_bit_cast(bf16x8, br), cr[i][j], 0, 0, 0);
+      ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, __builtin_bit_cast(bf16x8, bi), ci[i][j], 0, 0, 0);
+    };
 #pragma unroll
     for (int j = 0; j < TN; j++) {
-      const uint4 v0 = f.b0[j], v1 = f.b1[j];
-      // real part: (b_re, -b_im); imaginary part: (b_im, b_re)
-      auto ng = [](uint32_t w) { return w ^ 0x80000000u; };
-      auto rt = [](uint32_t w) { return __builtin_amdgcn_alignbit(w, w, 16u); };
-      const uint4 rh = make_uint4(ng(v0.x), ng(v0.z), ng(v1.x), ng(v1.z));
-      const uint4 rl = make_uint4(ng(v0.y), ng(v0.w), ng(v1.y), ng(v1.w));
-      const uint4 ih = make_uint4(rt(v0.x), rt(v0.z), rt(v1.x), rt(v1.z));
-      const uint4 il = make_uint4(rt(v0.y), rt(v0.w), rt(v1.y), rt(v1.w));
-#pragma unroll
-      for (int t = 0; t < 3; t++)
-#pragma unroll
-        for (int i = 0; i < TM; i++) {
-          const bf16x8 xa = __builtin_bit_cast(bf16x8, t == 2 ? f.al[i] : f.ah[i]);
-          const bf16x8 xr = __builtin_bit_cast(bf16x8, t == 1 ? rl : rh);
-          const bf16x8 xi = __builtin_bit_cast(bf16x8, t == 1 ? il : ih);
-          cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, xr, cr[i][j], 0, 0, 0);
-          ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, xi, ci[i][j], 0, 0, 0);
-        }
-      if constexpr (NP == 3) {
-        // h*l, l*h, m*m
-        const uint4 w = f.b3[j];
-        const uint4 r3 = make_uint4(ng(w.x), ng(w.y), ng(w.z), ng(w.w));
-        const uint4 i3 = make_uint4(rt(w.x), rt(w.y), rt(w.z), rt(w.w));
+      const uint4* t4 = (const uint4*)st;
+      const uint4 v0 = NP == 2 ? f.b0[j % TNF] : t4[(brow + 16 * j) * 8 + c0];
+      const uint4 v1 = NP == 2 ? f.b1[j % TNF] : t4[(brow + 16 * j) * 8 + c1];
+      if constexpr (NP == 2) {
+        const uint4 rh = make_uint4(ng(v0.x), ng(v0.z), ng(v1.x), ng(v1.z));
+        const uint4 rl = make_uint4(ng(v0.y), ng(v0.w), ng(v1.y), ng(v1.w));
+        const uint4 ih = make_uint4(rt(v0.x), rt(v0.z), rt(v1.x), rt(v1.z));
+        const uint4 il = make_uint4(rt(v0.y), rt(v0.w), rt(v1.y), rt(v1.w));
+        // h*h, h*m, m*h (the expanded kernel's order)
 #pragma unroll
         for (int t = 0; t < 3; t++)
 #pragma unroll
-          for (int i = 0; i < TM; i++) {
-            const bf16x8 xa = __builtin_bit_cast(bf16x8, t == 0 ? f.ah[i] : t == 1 ? f.a3[i] : f.al[i]);
-            const bf16x8 xr = __builtin_bit_cast(bf16x8, t == 0 ? r3 : t == 1 ? rh : rl);
-            const bf16x8 xi = __builtin_bit_cast(bf16x8, t == 0 ? i3 : t == 1 ? ih : il);
-            cr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, xr, cr[i][j], 0, 0, 0);
-            ci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xa, xi, ci[i][j], 0, 0, 0);
-          }
+          for (int i = 0; i < TM; i++) mm(i, j, t == 2 ? f.al[i] : f.ah[i], t == 1 ? rl : rh, t == 1 ? il : ih);
+      } else {
+        // grouped by B's piece so that one piece's two forms are live at a
+        // time: h*h, m*h, l*h, then h*m, m*m, then h*l
+        {
+          const uint4 rh = make_uint4(ng(v0.x), ng(v0.z), ng(v1.x), ng(v1.z));
+          const uint4 ih = make_uint4(rt(v0.x), rt(v0.z), rt(v1.x), rt(v1.z));
+#pragma unroll
+          for (int t = 0; t < 3; t++)
+#pragma unroll
+            for (int i = 0; i < TM; i++) mm(i, j, t == 0 ? f.ah[i] : t == 1 ? f.al[i] : f.a3[i], rh, ih);
+        }
+        {
+          const uint4 rl = make_uint4(ng(v0.y), ng(v0.w), ng(v1.y), ng(v1.w));
+          const uint4 il = make_uint4(rt(v0.y), rt(v0.w), rt(v1.y), rt(v1.w));
+#pragma unroll
+          for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int i = 0; i < TM; i++) mm(i, j, t == 0 ? f.ah[i] : f.al[i], rl, il);
+        }
+        {
+          const uint4 w = ((const uint4*)(st + (BM + BN) * KT))[(brow + 16 * j) * 4 + c3];
+          const uint4 r3 = make_uint4(ng(w.x), ng(w.y), ng(w.z), ng(w.w));
+          const uint4 i3 = make_uint4(rt(w.x), rt(w.y), rt(w.z), rt(w.w));
+#pragma unroll
+          for (int i = 0; i < TM; i++) mm(i, j, f.ah[i], r3, i3);
+        }
       }
     }
   };
@@ -254,7 +271,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
   // wait until tile c has landed (later tiles stay in flight), barrier (every
   // wave has also finished reading tile c - 1, whose stage tile j refills),
   // read the fragments, issue tile j, run the MFMAs.
-  auto step = [&](auto stc, int j) {
+  auto step = [&](auto stc, int j) __attribute__((always_inline)) {
     constexpr int ST = decltype(stc)::value;           // stage of tile j
     constexpr int SC = (ST + 1) % STAGES;              // stage of tile c
     uint2* in_ = ST == 0 ? s0 : ST == 1 ? s1 : ST == 2 ? s2 : s3;
@@ -272,7 +289,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
       read(cs, f);
     }
     if (j < nk) issue(in_, j * KT);
-    if (c >= 0) mfmas(f);
+    if (c >= 0) mfmas(f, cs);
   };
   for (int j = 0; j < nk + STAGES - 1; j += STAGES) {
     step(std::integral_constant<int, 0>(), j);
@@ -315,48 +332,36 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
         vr[j][q] = lc ? C[off] : (DT)0;
         vi[j][q] = lc ? C[off + 1] : (DT)0;
       }
-    if constexpr (OSPLIT) {
-      // column-major split output (the Taylor steps' transposed terms): the
-      // lane's four rows are 32 contiguous bytes, stored as two 16-byte words
-      if (CS && !D && p.scm == 1 && rbase + i * 16 + 3 < p.m) {
+    // column-major split output (the Taylor steps' transposed terms): the
+    // lane's four rows are 32 contiguous bytes, stored as two 16-byte words
+    bool vec = false;
+    if constexpr (OSPLIT) vec = CS && !CS3 && !D && p.scm == 1 && rbase + i * 16 + 3 < p.m;
+    if (vec) {
 #pragma unroll
         for (int j = 0; j < TN; j++) {
           const int col = cbase + j * 16;
           if (col >= p.n) continue;
-          zb v[4];
-          uint32_t v3[4];
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
+          auto piece = [&](int q) __attribute__((always_inline)) -> zb {
             const double re = (double)cr[i][j][q], im = (double)ci[i][j][q];
             const double ore = p.ar * re - p.ai * im, oim = p.ar * im + p.ai * re;
             const double f_r = (double)fr[j][q], f_i = (double)fi[j][q];
             const double sr = p.cf ? f_r + ore : ore, si = p.cf ? f_i + oim : oim;
-            if (CS3) split3_z((double)(float)sr, (double)(float)si, v[q], v3[q]);
-            else v[q] = split_z((float)sr, (float)si);
-          }
-          const long eoff = (long)(rbase + i * 16) + (long)col * p.scn;
-          zb* dz = CS + eoff;
+            return split_z((float)sr, (float)si);
+          };
+          const zb v0 = piece(0), v1 = piece(1), v2 = piece(2), v3 = piece(3);
+          zb* dz = CS + (long)(rbase + i * 16) + (long)col * p.scn;
           if (((uintptr_t)dz & 15) == 0) {
             uint4* dst = (uint4*)dz;
-            dst[0] = make_uint4(v[0].hi, v[0].lo, v[1].hi, v[1].lo);
-            dst[1] = make_uint4(v[2].hi, v[2].lo, v[3].hi, v[3].lo);
+            dst[0] = make_uint4(v0.hi, v0.lo, v1.hi, v1.lo);
+            dst[1] = make_uint4(v2.hi, v2.lo, v3.hi, v3.lo);
           } else {
-#pragma unroll
-            for (int q = 0; q < 4; q++) dz[q] = v[q];
-          }
-          if (CS3) {
-            uint32_t* d3 = CS3 + eoff;
-            if (((uintptr_t)d3 & 15) == 0) {
-              *(uint4*)d3 = make_uint4(v3[0], v3[1], v3[2], v3[3]);
-            } else {
-#pragma unroll
-              for (int q = 0; q < 4; q++) d3[q] = v3[q];
-            }
+            dz[0] = v0;
+            dz[1] = v1;
+            dz[2] = v2;
+            dz[3] = v3;
           }
         }
-        continue;
-      }
-    }
+    } else {
 #pragma unroll
     for (int j = 0; j < TN; j++)
 #pragma unroll
@@ -383,7 +388,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
             if (CS3) {
               zb h;
               uint32_t t;
-              split3_z((double)(float)sr, (double)(float)si, h, t);
+              split3_f((float)sr, (float)si, h, t);
               CS[off / 2] = h;
               CS3[off / 2] = t;
             } else {
@@ -395,5 +400,6 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
           C[off + 1] = (DT)(p.cf ? oci + f_i : oci);
         }
       }
+    }
   }
 }

@@ -291,6 +291,49 @@ __global__ void split3_kernel(const IT* in, zb* hm, uint32_t* __restrict__ l, lo
   }
 }
 
+// Third pieces of a split-BF16 array from its FP64 source: the stored (h, m)
+// pieces are taken as they are, l = bf16(x - h - m) per component.  Element o
+// of the split array (and of l) comes from src[(o % R) * C + o / R] when
+// trans (the split array is the [C][R] transpose of the [R][C] source), else
+// from src[o].
+__global__ void third_piece_kernel(const zd* __restrict__ src, const zb* __restrict__ hm, uint32_t* __restrict__ l,
+                                   long n, long R, long C, int trans)
+{
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < n; o += (long)gridDim.x * blockDim.x) {
+    const zd x = src[trans ? (o % R) * C + o / R : o];
+    const zb v = hm[o];
+    const double rr = x.re - (double)bf2f((unsigned short)(v.hi & 0xffff)) - (double)bf2f((unsigned short)(v.lo & 0xffff));
+    const double ri = x.im - (double)bf2f((unsigned short)(v.hi >> 16)) - (double)bf2f((unsigned short)(v.lo >> 16));
+    l[o] = (uint32_t)d2bf(rr) | ((uint32_t)d2bf(ri) << 16);
+  }
+}
+
+// Batched transpose into three-piece split-BF16: in [b][R][C] (row stride C,
+// item stride R*C) -> out [b][C][R] (zb) and its third pieces out3, per
+// 32 x 32 tile through LDS; grid (C/32, R/32, b).
+template <class IT>
+__global__ __launch_bounds__(256) void transpose_split3_kernel(const IT* __restrict__ in, zb* __restrict__ out,
+                                                               uint32_t* __restrict__ out3, int R, int C)
+{
+  __shared__ zd tile[32][33];
+  const long item = (long)blockIdx.z * R * C;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int rr = by + yy, cc = bx + tx;
+    if (rr < R && cc < C) tile[yy][tx] = to_zd(in[item + (long)rr * C + cc]);
+  }
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int cc = bx + yy, rr = by + tx;
+    if (rr < R && cc < C) {
+      const zd v = tile[tx][yy];
+      const long o = item + (long)cc * R + rr;
+      split3_z(v.re, v.im, out[o], out3[o]);
+    }
+  }
+}
+
 // W_data[w][2+s] = dcache[2w+s] (determinants cached by the last inverse of
 // the same walkers)
 __global__ void det_restore_kernel(zd* __restrict__ Wd, const zd* __restrict__ dcache, int nw)
@@ -610,7 +653,8 @@ __device__ inline void field_term(zd b, double xr, zd& h, zd& Xo)
 template <class XT>
 __global__ __launch_bounds__(256) void fields_wfirst_kernel(int K, int nw, const zd* __restrict__ vb, FieldSrc src,
                                                             XT* __restrict__ X, zd* __restrict__ hybridW,
-                                                            int* __restrict__ nonfinite)
+                                                            int* __restrict__ nonfinite,
+                                                            uint32_t* __restrict__ X3 = nullptr)
 {
   __shared__ zd part[4];
   const int w = blockIdx.x;
@@ -619,6 +663,17 @@ __global__ __launch_bounds__(256) void fields_wfirst_kernel(int K, int nw, const
   const double* x = src.x ? src.x + (src.xslice ? sub * (long)K * nw : 0L) : nullptr;
   const zd* vbw = vb + (long)w * K;
   XT* Xw = X + (long)w * K;
+  uint32_t* X3w = X3 ? X3 + (long)w * K : nullptr;
+  // split X with its third pieces (three-piece split-BF16), else as XT
+  auto put = [&](int n, zd o) {
+    if constexpr (std::is_same<XT, zb>::value) {
+      if (X3w) {
+        split3_z(o.re, o.im, Xw[n], X3w[n]);
+        return;
+      }
+    }
+    from_zd(o, Xw[n]);
+  };
   zd h{0, 0};
   bool bad = false;
   for (int n = 2 * threadIdx.x; n < K; n += 512) {
@@ -632,10 +687,10 @@ __global__ __launch_bounds__(256) void fields_wfirst_kernel(int K, int nw, const
     }
     zd o;
     field_term(vbw[n], xa, h, o);
-    from_zd(o, Xw[n]);
+    put(n, o);
     if (n + 1 < K) {
       field_term(vbw[n + 1], xb, h, o);
-      from_zd(o, Xw[n + 1]);
+      put(n + 1, o);
     }
   }
   if (bad) atomicOr(nonfinite, 1);

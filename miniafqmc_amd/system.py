@@ -38,7 +38,7 @@ class Csr:
     def dense(self):
         out = np.zeros((self.nrows, self.ncols), np.complex128)
         rows = np.repeat(np.arange(self.nrows), np.diff(self.rowptr))
-        out[rows, self.col] = self.val
+        np.add.at(out, (rows, self.col), self.val)  # repeated entries add up, as in csrmm
         return out
 
 

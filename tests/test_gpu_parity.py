@@ -585,3 +585,53 @@ def test_every_ring_tile_matches_register_path(require_gpu, tile):
         g.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b), f"tile {tile}: max diff {np.abs(a - b).max():.3e}"
+
+
+def test_repeated_spvn_entries_are_summed(require_gpu):
+    """A Spvn with repeated (row, column) entries (the reference keeps them:
+    SparseMatrix::compress only sorts, SparseMatrix.hpp:295-319, and csrmm
+    sums them, sparse.hpp:166-180) gives the walkers and energies of the same
+    matrix with each repeated entry summed, in every mode: every entry of the
+    first 40 rows is split into two halves, and row 7 repeats its first entry
+    with value zero."""
+    import dataclasses
+    from miniafqmc_amd.system import Csr
+    s = m.make_system(24, 6, 40, density=0.5, sigma_L=0.03, sigma_h=1.0, dt=0.005, seed=77, with_vakbl=True)
+    sp = s.spvn
+    rows, cols, vals = [], [], []
+    for r in range(sp.nrows):
+        for q in range(sp.rowptr[r], sp.rowptr[r + 1]):
+            if r < 40:
+                rows += [r, r]
+                cols += [sp.col[q], sp.col[q]]
+                vals += [sp.val[q] / 2, sp.val[q] / 2]
+            else:
+                rows.append(r)
+                cols.append(sp.col[q])
+                vals.append(sp.val[q])
+            if r == 7 and q == sp.rowptr[r]:
+                rows.append(r)
+                cols.append(sp.col[q])
+                vals.append(0.0)
+    rp = np.zeros(sp.nrows + 1, np.int32)
+    np.add.at(rp, np.asarray(rows) + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    dup = Csr(sp.nrows, sp.ncols, rp, np.asarray(cols, np.int32), np.asarray(vals, np.complex128))
+    assert np.array_equal(dup.dense(), sp.dense())
+    s2 = dataclasses.replace(s, spvn=dup)
+    nw, nsub = 6, 3
+    x = m.fields_block(77, s.nchol, nw, 0, nsub)
+    for mode, tol in (("fp64_csr", 1e-12), ("fp64_dense", 0.0), ("mixed_bf16", 0.0)):
+        out = []
+        for sysm in (s, s2):
+            g = m.AfqmcSys(sysm, nw, mode)
+            g.init_walkers()
+            g.stage_fields(x)
+            g.calculate_mixed_density_matrix()
+            g.run(1, nsub, 2, s.dt)
+            out.append((g.W, g.W_data))
+            g.close()
+        for a, b in zip(out[0], out[1]):
+            d = np.abs(a - b).max() / np.abs(a).max()
+            say(f"repeated Spvn entries, {mode}: max rel diff {d:.2e}")
+            assert d <= tol
