@@ -230,9 +230,15 @@ def main():
     ap.add_argument("--direct-lds", type=int, default=2,
                     help="split-BF16 GEMMs: 2 both split operands through a ring of LDS stages by direct loads, "
                          "1 the left operand by direct loads into two stages, 0 through registers")
-    ap.add_argument("--three-piece", type=int, default=0,
+    ap.add_argument("--three-piece", type=int, default=4,
                     help="split-BF16 mode: contractions in three-piece split-BF16 (24-bit operands), a mask: "
                          "1 bias product, 2 HS potential, 4 propagator")
+    ap.add_argument("--three-steps", type=lambda v: int(v, 0), default=0x81,
+                    help="with the propagator in three pieces: a mask of its eight products that take three "
+                         "pieces (bit 0 P W, bits 1-6 the Horner steps U_6 .. U_1, bit 7 P U)")
+    ap.add_argument("--variant", type=int, default=1,
+                    help="split-BF16 mode: also time the other piece count (two pieces if --three-piece is set, "
+                         "else all three-piece) from the same walkers and report it beside the value")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB and exit (reference for "
                          "hardware byte counters)")
@@ -245,13 +251,21 @@ def main():
 
     if args.calib_copy_mib > 0:
         # reference for hardware byte counters: one device-to-device copy of
-        # known size, in a process of its own (nothing else is run)
-        import torch
-        n = args.calib_copy_mib << 18  # float32 elements
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        # known size through the HIP runtime alone, in a process of its own
+        # (no other GPU library is loaded; the fill before it is a separate
+        # dispatch)
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = C.c_size_t(args.calib_copy_mib << 20)
+        src, dst = C.c_void_p(), C.c_void_p()
+        for ptr in (src, dst):
+            if hip.hipMalloc(C.byref(ptr), nbytes) != 0:
+                sys.exit("bench.py: hipMalloc failed")
+        ok = (hip.hipMemset(src, 1, nbytes) == 0 and hip.hipDeviceSynchronize() == 0
+              and hip.hipMemcpy(dst, src, nbytes, 3) == 0 and hip.hipDeviceSynchronize() == 0)  # 3: device to device
+        hip.hipFree(src)
+        hip.hipFree(dst)
+        if not ok:
+            sys.exit("bench.py: calibration copy failed")
         print(f"[bench] calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written",
               flush=True)
         return
@@ -299,13 +313,16 @@ def main():
             out[k] = g.W_data[:, 0]
         return out
 
-    def run_mode(mode, profile, state=None):
+    def run_mode(mode, profile, state=None, three=None):
         """Time `args.steps` substeps in `mode`; then run the accuracy probes
         from `state`, or from this run's own final walkers (returned)."""
+        three = args.three_piece if three is None else three
         g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level,
-                       three_piece=args.three_piece if mode == "mixed_bf16" else 0)
+                       three_piece=three if mode == "mixed_bf16" else 0)
         g.set_exchange_chunk(args.exchange_chunk)
         g.set_direct_lds(args.direct_lds)
+        if mode == "mixed_bf16" and three & 4:
+            g.set_three_piece_steps(args.three_steps)
         # the auxiliary fields are drawn on the device inside each substep
         # (the reference's generate_normal in the step loop), the same
         # counter-based draws the oracle is given on the host
@@ -441,6 +458,19 @@ def main():
                 "Eav_Ha": [float(eav[-1]), float(eav64[-1])], "substeps": args.warmup + args.steps,
             }
             extra["E_L_mean_Ha"] = float(Wd64[:, 0].real.mean())
+            # the other split-BF16 piece count on the same workload, from the
+            # same walkers: its speed and its errors against the same FP64 probes
+            if args.mode == "mixed_bf16" and args.variant:
+                alt = 0 if args.three_piece else 7
+                Tv, _, rv = run_mode(args.mode, False, state=res.get("state"), three=alt)
+                vv = nw * args.steps / Tv
+                extra["variant"] = {
+                    "three_piece": alt, "value": round(vv, 2), "ms_per_step": round(1e3 * Tv / args.steps, 3),
+                    "speedup_vs_fp64": round(vv / v64, 3),
+                    "energy_eval_error": err(rv["probe"]["eval"][0], e64),
+                    "step_error": {f"{k}_substeps": err(rv["probe"][k], p64[k]) for k in probe_steps}}
+                print(f"[bench] variant three_piece={alt}: {vv:.1f} walker-steps/s ({vv / v64:.2f}x fp64); "
+                      f"step_error {extra['variant']['step_error']}", flush=True)
         print(f"[bench] fp64_dense: {v64:.1f} walker-steps/s; mixed/fp64 = {value / v64:.2f}x", flush=True)
         for k_, (n_, ms) in sorted(r64["stats"].items(), key=lambda kv: -kv[1][1])[:12]:
             print(f"[bench]   fp64 {k_:20s} {n_:6d} launches {ms:10.2f} ms total {ms / max(n_, 1):9.3f} ms avg",
@@ -494,13 +524,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": MODE_DTYPE[args.mode],
+        # split-BF16 with some contractions in three pieces (six MFMAs a product)
+        "dtype": "bf16x3+bf16x6+c128" if args.mode == "mixed_bf16" and args.three_piece else MODE_DTYPE[args.mode],
         "data": "synthetic (seeded Hamiltonian and auxiliary fields)",
         "config": {"workload": args.workload, "nmo": M, "naea": N, "nchol": K, "walkers": nw, "dt": s.dt,
                    "mode": args.mode, "northo": northo, "energy_level": args.energy_level,
                    "exchange_chunk": args.exchange_chunk if args.energy_level < 3 else None,
                    "graphs": bool(args.graphs), "direct_lds": args.direct_lds,
                    "three_piece": args.three_piece if args.mode == "mixed_bf16" else 0,
+                   "three_piece_steps": args.three_steps if args.mode == "mixed_bf16" and args.three_piece & 4
+                   else None,
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -143,7 +143,13 @@ typedef enum {
    * 2 the HS potential, 4 the propagator (P W, the Taylor terms, P U).
    * Needs M, K and N*M multiples of 16.  Default 0.  No counterpart in the
    * reference. */
-  AFQMC_OPT_THREE_PIECE = 8
+  AFQMC_OPT_THREE_PIECE = 8,
+  /* with bit 4 of AFQMC_OPT_THREE_PIECE: a mask of the propagator's eight
+   * products that take three pieces, in launch order (bit 0 P W, bits 1-6
+   * the Horner steps U_6 .. U_1, bit 7 P U); the others take two pieces of
+   * the same operands.  Default 255 (all).  May be changed at any time.  No
+   * counterpart in the reference. */
+  AFQMC_OPT_THREE_PIECE_STEPS = 9
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

@@ -176,6 +176,13 @@ class AfqmcSys:
         the autotuned / heuristic choice."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_TILE, float(k)))
 
+    def set_three_piece_steps(self, mask):
+        """With the propagator in three pieces (bit 4 of `three_piece`): a
+        mask of its eight products that take three pieces, in launch order
+        (bit 0 P W, bits 1-6 the Horner steps U_6 .. U_1, bit 7 P U); the
+        others take two pieces.  255 (the default) is all."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_THREE_PIECE_STEPS, float(mask)))
+
     def set_exchange_chunk(self, cols):
         """Dense exchange: accumulate the exchange GEMM over chunks of `cols`
         columns, adding the chunks' energies in FP64 (0: one pass)."""

@@ -92,11 +92,17 @@ struct afqmc_ctx {
   zd* Wd = nullptr;
   zd* T1 = nullptr;       // [nw][2][N][N] overlap matrices, then their inverses
   zd* dcache = nullptr;   // [nw][2] determinants of the last inversion
+  // log-overlaps [nw][4]: log det of the current overlap matrices (spin 0,
+  // 1) and of the saved ones (W_data columns 6, 7); the weights take the
+  // overlap ratio from them
+  zd* Wl = nullptr;
+  zd* lcache = nullptr;   // [nw][2] log determinants of the last inversion
   zd* gjwork = nullptr;   // N > 128: [nw][2][N][N] working matrices of the batched inverse
   bool t1inv_valid = false;  // T1 holds the inverses for the current walkers
   bool g_fresh = false;      // G was built from T1 and W as they are now
   zd* Wbr = nullptr;         // population control: gathered walkers [nw][2][M][N]
   zd* Wdbr = nullptr;        // and W_data rows [nw][8]
+  zd* Wlbr = nullptr;        // and log-overlaps [nw][4]
   double* brcum = nullptr;   // cumulative weights [nw]
   int* brparent = nullptr;   // parent of each new walker [nw]
   double* brw = nullptr;     // common weight after the comb
@@ -146,6 +152,10 @@ struct afqmc_ctx {
   uint32_t* tp_vHS = nullptr;  // [nw][M*M] (propagator)
   uint32_t* tp_Ta = nullptr;   // [nw][2][N][M] (propagator terms)
   uint32_t* tp_Tb = nullptr;
+  // with bit 4 of `three`: which of the propagator's eight products take
+  // three pieces (bit 0 P W, bit 7-k the Horner step k = 6..1, bit 7 P U);
+  // the others take two pieces of the same split operands
+  int three_steps = 0xFF;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
   void* vws = nullptr;    // split-k partials of the bias contraction
@@ -593,7 +603,8 @@ int csrmm_n(afqmc_ctx* c, const char* name, int nrows, int K, long long nnz, con
 // Batched inverse + determinant of the nw*2 overlap matrices T (in place
 // when Tinv == T); det of matrix b at det[(b/2)*dq + (b%2)*dr], a count of
 // singular matrices in *flags and the determinants in dc when those are set.
-int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, int* flags, zd* dc)
+int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, int* flags, zd* dc, zd* ldet,
+                    zd* lc)
 {
   const int N = c->N;
   const long batch = 2L * c->nw;
@@ -601,22 +612,22 @@ int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, in
   if (N <= 16)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<16, 4>), dim3(batch), dim3(64), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         flags, dc);
+                         flags, dc, ldet, lc);
     });
   if (N <= 32)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<32, 4>), dim3(batch), dim3(128), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         flags, dc);
+                         flags, dc, ldet, lc);
     });
   if (N <= 64)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<64, 4>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
-                         flags, dc);
+                         flags, dc, ldet, lc);
     });
   if (N <= 128)
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL((gj_inverse_kernel<128, 4>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N,
-                         ms, flags, dc);
+                         ms, flags, dc, ldet, lc);
     });
   if (N <= GJ_GMAX) {
     // the working matrices do not fit one workgroup's registers: they stay
@@ -624,7 +635,7 @@ int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, in
     if (!c->gjwork) RET(dalloc(c, &c->gjwork, (size_t)batch * ms));
     return launch(c, "gj_inverse", [&]() {
       hipLaunchKernelGGL(gj_inverse_global_kernel, dim3(batch), dim3(1024), 0, c->stream, T, Tinv, c->gjwork, det, dq,
-                         dr, N, ms, flags, dc);
+                         dr, N, ms, flags, dc, ldet, lc);
     });
   }
   return fail(c, AFQMC_ERR_UNSUPPORTED, "N > 256 electrons per spin is not supported by the batched inverse");
@@ -633,7 +644,7 @@ int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, in
 // ... of the context's overlap matrices, with its flags and determinant cache
 int gj_inverse(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr)
 {
-  return gj_inverse_into(c, T, Tinv, det, dq, dr, c->flags, c->dcache);
+  return gj_inverse_into(c, T, Tinv, det, dq, dr, c->flags, c->dcache, c->Wl, c->lcache);
 }
 
 // T1[w][s] = W[w][s]^T * trial_s  (N x N, k = M), all walkers and spins
@@ -719,7 +730,7 @@ int do_mdm(afqmc_ctx* c)
     // the overlap step already inverted W^T conj(Psi) for these walkers
     RET(launch(c, "det_restore", [&]() {
       hipLaunchKernelGGL(det_restore_kernel, dim3((2 * nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->dcache,
-                         nw);
+                         c->Wl, c->lcache, nw);
     }));
   } else {
     RET(overlap_matrices(c));
@@ -1064,13 +1075,18 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
 
 // The propagator in three-piece split-BF16: W and S are transposed into
 // three-piece terms ([N][M] per walker and spin) so that every product reads
-// both operands k-contiguous through the staged-ring kernel; the Horner terms
-// are written with their third pieces.
+// both operands k-contiguous through the staged-ring kernel.  Each product
+// runs in three pieces or, where `three_steps` says so, in two (the (h, m)
+// words of a three-piece operand are its two-piece split); a Horner term is
+// written with its third pieces only when the product that reads it takes
+// three.
 int propagate_three(afqmc_ctx* c)
 {
   const int M = c->M, N = c->N, nw = c->nw;
   const long MN = (long)M * N, MM = (long)M * M;
   const dim3 tgrid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32), (unsigned)(2 * nw));
+  // product j in launch order: 0 P W, 7-k the Horner step k, 7 P U
+  auto three = [&](int j) { return ((c->three_steps >> j) & 1) != 0; };
   zb* Ta = (zb*)c->Ta;
   zb* Tb = (zb*)c->Tb;
   RET(launch(c, "prop_split_w", [&]() {
@@ -1101,7 +1117,8 @@ int propagate_three(afqmc_ctx* c)
     a.scn = 1;
     a.sc1 = 2 * MN;
     a.sc2 = MN;
-    RET((zgemm<CP_BF16X6, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
+    if (three(0)) RET((zgemm<CP_BF16X6, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
+    else RET((zgemm<CP_BF16X3, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
   }
   RET(launch(c, "prop_split_s", [&]() {
     hipLaunchKernelGGL(transpose_split3_kernel<zf>, tgrid, dim3(256), 0, c->stream, (const zf*)c->S, Tb, c->tp_Tb, M,
@@ -1121,7 +1138,7 @@ int propagate_three(afqmc_ctx* c)
     a.B = prev;
     a.B3 = prev3;
     a.C = bufs[k & 1];
-    a.C3 = bufs3[k & 1];
+    a.C3 = three(8 - k) ? bufs3[k & 1] : nullptr;
     a.scm = 1;
     a.scn = M;
     a.sc1 = 2 * MN;
@@ -1132,7 +1149,8 @@ int propagate_three(afqmc_ctx* c)
     a.sfm = N;
     a.sfn = 1;
     a.cf = 1;
-    RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
+    if (three(7 - k)) RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
+    else RET((zgemm<CP_BF16X3, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
     prev = bufs[k & 1];
     prev3 = bufs3[k & 1];
   }
@@ -1145,7 +1163,8 @@ int propagate_three(afqmc_ctx* c)
     a.scn = 1;
     a.sc1 = 2 * MN;
     a.sc2 = MN;
-    RET((zgemm<CP_BF16X6, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
+    if (three(7)) RET((zgemm<CP_BF16X6, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
+    else RET((zgemm<CP_BF16X3, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
   }
   return AFQMC_OK;
 }
@@ -1196,7 +1215,8 @@ int do_propagate(afqmc_ctx* c)
 int do_save_overlaps(afqmc_ctx* c)
 {
   return launch(c, "save_overlaps", [&]() {
-    hipLaunchKernelGGL(save_overlaps_kernel, dim3((c->nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->nw);
+    hipLaunchKernelGGL(save_overlaps_kernel, dim3((c->nw + 255) / 256), dim3(256), 0, c->stream, c->Wd, c->Wl,
+                       c->nw);
   });
 }
 
@@ -1212,6 +1232,7 @@ int do_branch(afqmc_ctx* c, double u, int* parent_out)
   if (!c->Wbr) {
     RET(dalloc(c, &c->Wbr, (size_t)nw * per));
     RET(dalloc(c, &c->Wdbr, (size_t)nw * 8));
+    RET(dalloc(c, &c->Wlbr, (size_t)nw * 4));
     RET(dalloc(c, &c->brcum, (size_t)nw));
     RET(dalloc(c, &c->brparent, (size_t)nw));
     RET(dalloc(c, &c->brw, 1));
@@ -1229,10 +1250,12 @@ int do_branch(afqmc_ctx* c, double u, int* parent_out)
   const unsigned grid = (unsigned)std::min<long>(16384, std::max<long>(1, (n + 255) / 256));
   RET(launch(c, "branch_gather", [&]() {
     hipLaunchKernelGGL(comb_gather_kernel, dim3(grid), dim3(256), 0, c->stream, (const zd*)c->W, c->Wbr,
-                       (const zd*)c->Wd, c->Wdbr, (const int*)c->brparent, (const double*)c->brw, per, nw);
+                       (const zd*)c->Wd, c->Wdbr, (const zd*)c->Wl, c->Wlbr, (const int*)c->brparent,
+                       (const double*)c->brw, per, nw);
   }));
   HIPCHK(c, hipMemcpyAsync(c->W, c->Wbr, sizeof(zd) * n, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->Wd, c->Wdbr, sizeof(zd) * nw * 8, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->Wl, c->Wlbr, sizeof(zd) * nw * 4, hipMemcpyDeviceToDevice, c->stream));
   c->t1inv_valid = false;
   c->g_fresh = false;
   if (parent_out) {
@@ -1248,7 +1271,8 @@ int do_weights(afqmc_ctx* c, const double* dtp, double dt)
   int th = 64;
   while (th < c->nw && th < 1024) th *= 2;  // power of two for the tree reduction
   return launch(c, "weights", [&]() {
-    hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(th), 0, c->stream, c->Wd, c->hybridW, c->nw, dt, c->eshift);
+    hipLaunchKernelGGL(weights_kernel, dim3(1), dim3(th), 0, c->stream, c->Wd, c->Wl, c->hybridW, c->nw, dt,
+                       c->eshift);
   });
 }
 
@@ -1836,6 +1860,8 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   chk(dalloc(c, &c->Wd, nw * 8));
   chk(dalloc(c, &c->T1, nw * 2 * N * N));
   chk(dalloc(c, &c->dcache, nw * 2));
+  chk(dalloc(c, &c->lcache, nw * 2));
+  chk(dalloc(c, &c->Wl, nw * 4));
   chk(dalloc_bytes(c, &c->G, nw * 2 * MN * zs));
   chk(dalloc(c, &c->vbias, nw * K));
   chk(dalloc_bytes(c, &c->X, nw * K * zs));
@@ -1861,6 +1887,8 @@ int afqmc_ctx_create(int nmo, int naea, int nchol, int nwalk, int mode, afqmc_ct
   (void)hipMemsetAsync(c->eshift, 0, sizeof(double), c->stream);
   (void)hipMemsetAsync(c->flags, 0, 4 * sizeof(int), c->stream);
   (void)hipMemsetAsync(c->Wd, 0, sizeof(zd) * nw * 8, c->stream);
+  hipLaunchKernelGGL(wd_logs_kernel, dim3((unsigned)((4L * nw + 255) / 256)), dim3(256), 0, c->stream, c->Wd, c->Wl,
+                     nw);
   *out = c;
   return AFQMC_OK;
 }
@@ -1930,6 +1958,13 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     if (value != 0 && c->mode != AFQMC_MIXED_BF16)
       return fail(c, AFQMC_ERR_UNSUPPORTED, "three-piece contractions are a split-BF16 option");
     c->three = (int)value;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_THREE_PIECE_STEPS) {
+    if (value < 0 || value > 255 || value != (int)value)
+      return fail(c, AFQMC_ERR_ARG, "three-piece propagator step mask is 0..255");
+    drop_graphs(c);
+    c->three_steps = (int)value;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_TILE) {
@@ -2287,7 +2322,7 @@ int afqmc_init_walkers(afqmc_ctx* c)
   long tot = std::max<long>((long)c->nw * 2 * MN, (long)c->nw * 8);
   return launch(c, "init_walkers", [&]() {
     hipLaunchKernelGGL(init_walkers_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, c->W, c->Wd,
-                       c->trial, c->nw, MN);
+                       c->Wl, c->trial, c->nw, MN);
   });
 }
 
@@ -2363,7 +2398,14 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
   if (to_device && (buffer == AFQMC_BUF_W || buffer == AFQMC_BUF_G)) c->g_fresh = false;
   switch (buffer) {
   case AFQMC_BUF_W: return raw(c->W, sizeof(zd) * nw * 2 * MN);
-  case AFQMC_BUF_WDATA: return raw(c->Wd, sizeof(zd) * nw * 8);
+  case AFQMC_BUF_WDATA:
+    RET(raw(c->Wd, sizeof(zd) * nw * 8));
+    if (to_device)  // the log-overlaps follow the overlaps written
+      RET(launch(c, "wdata_logs", [&]() {
+        hipLaunchKernelGGL(wd_logs_kernel, dim3((unsigned)((4 * nw + 255) / 256)), dim3(256), 0, st, c->Wd, c->Wl,
+                           (int)nw);
+      }));
+    return AFQMC_OK;
   case AFQMC_BUF_HYBRIDW: return raw(c->hybridW, sizeof(zd) * nw);
   case AFQMC_BUF_G: return d ? wlast(c->G, NAK, c->mixed, c->tp_G) : raw(c->G, sizeof(zd) * NAK * nw);
   case AFQMC_BUF_VBIAS: return d ? wlast(c->vbias, K, false) : raw(c->vbias, sizeof(zd) * K * nw);
@@ -2800,7 +2842,7 @@ extern "C" int afqmc_mixed_density_matrix_noalloc(int nmo, int naea, int nwalk, 
   int r = zgemm_run<CP_F64, double, double, double, false, false>(c, "ovlp_gemm", a, 2 * nwalk,
                                                                    default_cfg<CP_F64>(a));
   // in-place inverse and determinant (no status flag: nothing is read back)
-  if (r == AFQMC_OK) r = gj_inverse_into(c, T1, T1, (zd*)ovlp, 2, 1, nullptr, nullptr);
+  if (r == AFQMC_OK) r = gj_inverse_into(c, T1, T1, (zd*)ovlp, 2, 1, nullptr, nullptr, nullptr, nullptr);
   if (r == AFQMC_OK) {
     // Gc[(s N + a) M + k][w] = sum_b T1inv(a, b) W(k, b)
     ZGemmArgs g = gargs();

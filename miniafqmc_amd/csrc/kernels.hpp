@@ -20,15 +20,51 @@
 // each thread updates its own entries.
 // batch b: input T + b*mstride; det written to det[(b/2)*dq + (b%2)*dr] and
 // to dcache[b] when dcache != null.
+// log det = sum_k log(pv_k), principal logs, imaginary part wrapped to
+// [-pi, pi], from the signed pivots pv[0..N) in LDS; summed by one wave in a
+// fixed order (every lane returns the same value).  The overlap ratios of
+// the weights are taken from these logs, so that a determinant that
+// underflows or overflows as a product of N pivots keeps its ratio.
+__device__ inline zd log_det_sum(const zd* pv, int N)
+{
+  const int lane = threadIdx.x & 63;
+  double lr = 0.0, li = 0.0;
+  for (int q = lane; q < N; q += 64) {
+    const zd v = pv[q];
+    lr += log(hypot(v.re, v.im));
+    li += atan2(v.im, v.re);
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    lr += __shfl_xor(lr, off);
+    li += __shfl_xor(li, off);
+  }
+  return zd{lr, remainder(li, 6.283185307179586476925286766559)};
+}
+
+// log det of matrix b (log_det_sum of its signed pivots, {-inf, 0} when
+// singular) to ldet[(b/2)*4 + b%2] and lcache[b] where those are set;
+// called by every thread after the elimination.
+__device__ inline void store_log_det(const zd* pv, int N, bool singular, long b, zd* ldet, zd* lcache)
+{
+  if ((!ldet && !lcache) || threadIdx.x >= 64) return;
+  const zd L = singular ? zd{-INFINITY, 0.0} : log_det_sum(pv, N);
+  if (threadIdx.x == 0) {
+    if (ldet) ldet[(b / 2) * 4 + (b % 2)] = L;
+    if (lcache) lcache[b] = L;
+  }
+}
+
 template <int NMAX, int CG>
 __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
                                                                zd* __restrict__ det, long dq, long dr, int N,
                                                                long mstride, int* __restrict__ info,
-                                                               zd* __restrict__ dcache)
+                                                               zd* __restrict__ dcache, zd* __restrict__ ldet,
+                                                               zd* __restrict__ lcache)
 {
   constexpr int NC = NMAX / CG;
   __shared__ double red[NMAX];
-  __shared__ zd prow[NMAX], oldk[NMAX], colk[NMAX];
+  __shared__ zd prow[NMAX], oldk[NMAX], colk[NMAX], pv[NMAX];
   __shared__ int perm[NMAX], dest[NMAX];
   const int tid = threadIdx.x;
   const int i = tid / CG, cg = tid % CG;
@@ -77,6 +113,7 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
       for (int t = 0; t < NC; t++) oldk[cg + CG * t] = x[t];
     }
     if (cg == ck) colk[i] = xk;
+    if (cg == ck && i == p) pv[k] = (p != k) ? zd{-xk.re, -xk.im} : xk;  // signed pivot
     if (tid == 0) perm[k] = p;
     __syncthreads();
     const zd piv = prow[k];
@@ -119,6 +156,7 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
     det[(b / 2) * dq + (b % 2) * dr] = dt;
     if (dcache) dcache[b] = dt;
   }
+  store_log_det(pv, N, singular, b, ldet, lcache);
   if (!Tinv) return;
   if (singular) {
     // leave a defined value: zeros
@@ -165,10 +203,11 @@ constexpr int GJ_GMAX = 256;
 __global__ __launch_bounds__(1024) void gj_inverse_global_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
                                                                   zd* __restrict__ work, zd* __restrict__ det,
                                                                   long dq, long dr, int N, long mstride,
-                                                                  int* __restrict__ info, zd* __restrict__ dcache)
+                                                                  int* __restrict__ info, zd* __restrict__ dcache,
+                                                                  zd* __restrict__ ldet, zd* __restrict__ lcache)
 {
   __shared__ double red[GJ_GMAX];
-  __shared__ zd prow[GJ_GMAX], oldk[GJ_GMAX], colk[GJ_GMAX];
+  __shared__ zd prow[GJ_GMAX], oldk[GJ_GMAX], colk[GJ_GMAX], pv[GJ_GMAX];
   __shared__ int perm[GJ_GMAX], dest[GJ_GMAX];
   __shared__ int piv_row;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -221,6 +260,7 @@ __global__ __launch_bounds__(1024) void gj_inverse_global_kernel(const zd* __res
       break;
     }
     dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
+    if (tid == 0) pv[k] = (p != k) ? zd{-piv.re, -piv.im} : piv;  // signed pivot
     const zd ipv = zdiv(zd{1.0, 0.0}, piv);
     for (long e = tid; e < NN; e += 1024) {
       const int j = (int)(e / N), i = (int)(e % N);
@@ -249,6 +289,7 @@ __global__ __launch_bounds__(1024) void gj_inverse_global_kernel(const zd* __res
     det[(b / 2) * dq + (b % 2) * dr] = dt;
     if (dcache) dcache[b] = dt;
   }
+  store_log_det(pv, N, singular, b, ldet, lcache);
   if (!Tinv) return;
   zd* Ob = Tinv + b * mstride;
   if (singular) {
@@ -336,10 +377,25 @@ __global__ __launch_bounds__(256) void transpose_split3_kernel(const IT* __restr
 
 // W_data[w][2+s] = dcache[2w+s] (determinants cached by the last inverse of
 // the same walkers)
-__global__ void det_restore_kernel(zd* __restrict__ Wd, const zd* __restrict__ dcache, int nw)
+__global__ void det_restore_kernel(zd* __restrict__ Wd, const zd* __restrict__ dcache, zd* __restrict__ Wl,
+                                   const zd* __restrict__ lcache, int nw)
 {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < 2 * nw) Wd[(long)(b / 2) * 8 + 2 + (b % 2)] = dcache[b];
+  if (b < 2 * nw) {
+    Wd[(long)(b / 2) * 8 + 2 + (b % 2)] = dcache[b];
+    Wl[(long)(b / 2) * 4 + (b % 2)] = lcache[b];
+  }
+}
+
+// log-overlaps [nw][4] (current spin 0, 1; saved spin 0, 1) taken from the
+// overlaps a caller wrote into W_data (columns 2, 3, 6, 7)
+__global__ void wd_logs_kernel(const zd* __restrict__ Wd, zd* __restrict__ Wl, int nw)
+{
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 4L * nw) return;
+  const long w = e / 4, q = e % 4;
+  const zd v = Wd[w * 8 + (q < 2 ? 2 + q : 4 + q)];
+  Wl[e] = zd{log(hypot(v.re, v.im)), atan2(v.im, v.re)};
 }
 
 // ---------------------------------------------------------------------------
@@ -747,8 +803,9 @@ __global__ __launch_bounds__(1024) void fields_wlast_kernel(int K, int nw, const
   }
 }
 
-// W_data[:,5] = W_data[:,4]; W_data[:,6..7] = W_data[:,2..3]
-__global__ void save_overlaps_kernel(zd* __restrict__ Wd, int nw)
+// W_data[:,5] = W_data[:,4]; W_data[:,6..7] = W_data[:,2..3]; the
+// log-overlaps likewise
+__global__ void save_overlaps_kernel(zd* __restrict__ Wd, zd* __restrict__ Wl, int nw)
 {
   int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw) return;
@@ -756,22 +813,29 @@ __global__ void save_overlaps_kernel(zd* __restrict__ Wd, int nw)
   d[5] = d[4];
   d[6] = d[2];
   d[7] = d[3];
+  zd* l = Wl + (long)w * 4;
+  l[2] = l[0];
+  l[3] = l[1];
 }
 
 // weights / hybrid energy / Eshift (src/miniapps/miniafqmc_base.cpp:317-327).
+// The overlap ratio W_data[2]W_data[3] / (W_data[6]W_data[7]) enters as its
+// log, from the log-overlaps Wl (log |ratio| and its phase, wrapped to
+// [-pi, pi]), so that overlaps beyond the FP64 range keep their ratio.
 // One block of up to 1024 threads; fixed-order tree reduction for Eshift.
-__global__ __launch_bounds__(1024) void weights_kernel(zd* __restrict__ Wd, const zd* __restrict__ hybridW, int nw,
-                                                       double dt, double* __restrict__ eshift)
+__global__ __launch_bounds__(1024) void weights_kernel(zd* __restrict__ Wd, const zd* __restrict__ Wl,
+                                                       const zd* __restrict__ hybridW, int nw, double dt,
+                                                       double* __restrict__ eshift)
 {
   __shared__ double red[1024];
   const double Es = *eshift;
   double et = 0.0;
   for (int w = threadIdx.x; w < nw; w += blockDim.x) {
     zd* d = Wd + (long)w * 8;
-    zd ratio = zdiv(zmul(d[2], d[3]), zmul(d[6], d[7]));
-    double ang = atan2(ratio.im, ratio.re);
+    const zd* l = Wl + (long)w * 4;
+    const double lr = (l[0].re + l[1].re) - (l[2].re + l[3].re);
+    const double ang = remainder((l[0].im + l[1].im) - (l[2].im + l[3].im), 6.283185307179586476925286766559);
     double scale = fmax(0.0, cos(ang));
-    double lr = log(hypot(ratio.re, ratio.im));
     zd h = hybridW[w];
     d[4] = zd{-(h.re + lr) / dt, -(h.im + ang) / dt};
     double f = scale * exp(-dt * (0.5 * (d[4].re + d[5].re) - Es));
@@ -834,8 +898,8 @@ __global__ __launch_bounds__(1024) void comb_select_kernel(const zd* __restrict_
 // Walkers and W_data rows of the selected parents: Wout[j] = W[parent[j]],
 // Wdout[j] = Wd[parent[j]] with the weight replaced by the common weight.
 __global__ void comb_gather_kernel(const zd* __restrict__ W, zd* __restrict__ Wout, const zd* __restrict__ Wd,
-                                   zd* __restrict__ Wdout, const int* __restrict__ parent,
-                                   const double* __restrict__ wnew, long per, int nw)
+                                   zd* __restrict__ Wdout, const zd* __restrict__ Wl, zd* __restrict__ Wlout,
+                                   const int* __restrict__ parent, const double* __restrict__ wnew, long per, int nw)
 {
   const long n = (long)nw * per;
   for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
@@ -846,11 +910,14 @@ __global__ void comb_gather_kernel(const zd* __restrict__ W, zd* __restrict__ Wo
     const long j = e / 8, k = e % 8;
     Wdout[e] = (k == 1) ? zd{*wnew, 0.0} : Wd[(long)parent[j] * 8 + k];
   }
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < (long)nw * 4; e += (long)gridDim.x * blockDim.x)
+    Wlout[e] = Wl[(long)parent[e / 4] * 4 + e % 4];
 }
 
-// W = conj(trial) per spin, W_data row = (0, 1, 0, ...)
-__global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, const zd* __restrict__ trial, int nw,
-                                    long MN)
+// W = conj(trial) per spin, W_data row = (0, 1, 0, ...), log-overlaps
+// log 0 = {-inf, 0}
+__global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, zd* __restrict__ Wl,
+                                    const zd* __restrict__ trial, int nw, long MN)
 {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long tot = (long)nw * 2 * MN;
@@ -859,6 +926,7 @@ __global__ void init_walkers_kernel(zd* __restrict__ W, zd* __restrict__ Wd, con
     W[i] = zconj(trial[rem]);
   }
   if (i < (long)nw * 8) Wd[i] = zd{(i % 8 == 1) ? 1.0 : 0.0, 0.0};
+  if (i < (long)nw * 4) Wl[i] = zd{-INFINITY, 0.0};
 }
 
 // ---------------------------------------------------------------------------

@@ -107,19 +107,24 @@ MIXED_DE_L = 1e-5  # Ha, |dE_L| target of the mixed path
 @pytest.mark.timeout(900)
 def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
     """The benchmark's configuration (split-BF16 contractions and propagator,
-    FP64 LU/overlaps/energy sums, FP32 exchange in chunks of 2048, fields
-    drawn on the device) against the FP64 oracle, and the FP64 dense path
-    against the oracle at 1e-10: one substep from the trial with the same
-    draws (walkers 0-1 of the 1024), every phase of the substep and the
-    local energy of the propagated walkers (afqmc_sys.hpp:95-133,
-    vbias.hpp:45-83, vHS.hpp:43-91, energy.hpp:63-102)."""
+    P W and P U with three-piece operands, FP64 LU/overlaps/energy sums,
+    FP32-precision exchange in chunks of 2048, fields drawn on the device)
+    and the same with two pieces throughout against the FP64 oracle, and the
+    FP64 dense path against the oracle at 1e-10: one substep from the trial
+    with the same draws (walkers 0-1 of the 1024), every phase of the
+    substep and the local energy of the propagated walkers
+    (afqmc_sys.hpp:95-133, vbias.hpp:45-83, vHS.hpp:43-91,
+    energy.hpp:63-102)."""
     s, p = m.preset_system("N128_M512")
     nw, seed, K = p["nwalk"], p["seed"], s.nchol
     M, N = s.nmo, s.naea
     gpu = {}
-    for mode in ("mixed_bf16", "fp64_dense"):
-        say(f"config 512 orbitals: GPU {mode}, {nw} walkers, one substep from the trial")
-        g = m.AfqmcSys(s, nw, mode, energy_precision=2)
+    for key, mode, three in (("mixed_bf16", "mixed_bf16", 0), ("mixed_bf16 P W, P U three-piece", "mixed_bf16", 4),
+                             ("fp64_dense", "fp64_dense", 0)):
+        say(f"config 512 orbitals: GPU {key}, {nw} walkers, one substep from the trial")
+        g = m.AfqmcSys(s, nw, mode, energy_precision=2, three_piece=three)
+        if three:
+            g.set_three_piece_steps(0x81)
         g.set_exchange_chunk(2048)
         g.device_fields(seed)
         g.init_walkers()
@@ -131,7 +136,7 @@ def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
         g.calculate_energy()
         r["E"] = g.W_data[:2, 0]
         r["terms"] = g.energy_terms()[:2]
-        gpu[mode] = r
+        gpu[key] = r
         g.close()
     say("config 512 orbitals: oracle on walkers 0-1")
     W0 = np.zeros((2, 2, M, N), complex)
@@ -155,7 +160,8 @@ def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
     say("config 512 orbitals: oracle local energy (Cholesky form)")
     orc.energy_cholesky(s, Wd1, G1, Lt)
     ref = {"G": G0, "vbias": vb, "X": X, "vHS": vhs, "hybridW": hyb, "W": W1}
-    for mode, bounds in (("fp64_dense", None), ("mixed_bf16", MIXED_BOUNDS)):
+    for mode, bounds in (("fp64_dense", None), ("mixed_bf16", MIXED_BOUNDS),
+                         ("mixed_bf16 P W, P U three-piece", MIXED_BOUNDS)):
         r = gpu[mode]
         for k_, v in ref.items():
             for w in range(2):

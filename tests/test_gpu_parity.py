@@ -635,3 +635,71 @@ def test_repeated_spvn_entries_are_summed(require_gpu):
             d = np.abs(a - b).max() / np.abs(a).max()
             say(f"repeated Spvn entries, {mode}: max rel diff {d:.2e}")
             assert d <= tol
+
+
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense", "mixed_bf16"])
+def test_log_overlaps_keep_ratios_beyond_fp64_range(c1, mode):
+    """The substep is invariant under scaling the walkers by a constant: G,
+    the bias, vHS and the propagation are, and the weights take the overlap
+    ratio from log-determinants.  Walkers scaled by 2^-73 (about 1e-22; a
+    power of two, so every product scales without rounding) have overlap
+    determinants of about 1e-352 per spin, zero as FP64 products, and must
+    give the same weights, hybrid energies, Eshift and orthogonalised
+    walkers as the unscaled ones; what differs is the rounding of the
+    log-determinant sums (about 1e3 each here)."""
+    s, p = c1
+    nw, nsub = 8, 4
+    W = perturbed_walkers(s, nw, seed=7, noise=0.05)
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, nsub)
+    out = []
+    for scale in (1.0, 2.0 ** -73):
+        say(f"log overlaps {mode} scale {scale:g}")
+        g = m.AfqmcSys(s, nw, mode)
+        g.stage_fields(x)
+        Wd = np.zeros((nw, 8), complex)
+        Wd[:, 1] = 1.0
+        g.set(nat.BUF_W, W * scale)
+        g.set(nat.BUF_WDATA, Wd)
+        g.calculate_mixed_density_matrix()
+        raw = g.W_data[:, 2:4].copy()
+        for t in range(nsub):
+            g.substep(t, t == nsub - 1, s.dt)
+        out.append((raw, g.W_data, g.W, g.eshift()))
+        g.close()
+    (raw1, wd1, w1, es1), (raws, wds, ws, ess) = out
+    assert np.all(np.abs(raw1) > 0)
+    assert np.all(raws == 0), "the scaled overlaps should underflow as FP64 products"
+    assert np.all(np.isfinite(wds[:, [1, 4, 5]]))
+    tol = 1e-10
+    close(wds[:, 1], wd1[:, 1], tol, "weights")
+    close(wds[:, 4:6], wd1[:, 4:6], tol, "hybrid energies")
+    close(ws, w1, tol, "orthogonalised walkers")
+    assert abs(ess - es1) <= tol * abs(es1)
+
+
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense", "mixed_bf16"])
+@pytest.mark.parametrize("graphs", [False, True])
+def test_device_fields_equal_host_draws(c1, mode, graphs):
+    """Fields drawn on the device inside the substep (afqmc_device_fields)
+    are the host generator's draws bit for bit: the same X, hybrid weights
+    and walkers as fields staged from the host (miniafqmc_base.cpp:282)."""
+    s, p = c1
+    nw, nsub = 12, 3
+    out = []
+    for device in (False, True):
+        say(f"device fields {mode} graphs={graphs} device={device}")
+        g = m.AfqmcSys(s, nw, mode)
+        if device:
+            g.device_fields(p["seed"])
+        else:
+            g.stage_fields(m.fields_block(p["seed"], s.nchol, nw, 0, nsub))
+        g.init_walkers()
+        g.calculate_mixed_density_matrix()
+        if graphs:
+            g.use_graphs(True)
+        for t in range(nsub):
+            g.substep(t, t == 2, s.dt)
+        out.append((g.get(nat.BUF_X), g.get(nat.BUF_HYBRIDW), g.W, g.W_data))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

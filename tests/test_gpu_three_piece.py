@@ -142,3 +142,63 @@ def test_three_piece_option_checks(c1):
     with pytest.raises(nat.AfqmcError):
         nat.check(nat.lib.afqmc_set_option(g.ctx, nat.OPT_THREE_PIECE, 7.0), g.ctx)
     g.close()
+
+
+def propagate_error(s, p, mask):
+    """Relative error of the propagated walkers against the oracle, with the
+    propagator in three pieces on the products `mask` selects (bit 0 P W,
+    bits 1-6 the Horner steps U_6 .. U_1, bit 7 P U)."""
+    nw = p["nwalk"]
+    g = m.AfqmcSys(s, nw, "mixed_bf16", three_piece=4)
+    g.set_three_piece_steps(mask)
+    W = perturbed_walkers(s, nw, seed=11)
+    spT = orc.halfrotate(s, cutoff=-1.0)
+    Wd = np.zeros((nw, 8), complex)
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    Xo, _ = orc.fields(orc.get_vbias(spT, Go), m.fields(p["seed"], s.nchol, nw, 0))
+    vhso = orc.get_vhs(s.spvn, Xo)
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_VHS, vhso)
+    g.propagate()
+    e = rel(g.W, orc.propagate(s, W, vhso))
+    g.close()
+    return e
+
+
+def test_three_piece_propagator_step_masks(c1):
+    """The propagator's products take three pieces where the step mask says:
+    all eight (255, the default) meet the three-piece bound, none (0) sits at
+    the two-piece level, and P W and P U alone (0x81) lie between."""
+    s, p = c1
+    e = {mk: propagate_error(s, p, mk) for mk in (0x00, 0x81, 0xC1, 0xFF)}
+    say("propagator step masks: " + ", ".join(f"0x{k:02x} {v:.2e}" for k, v in e.items()))
+    assert e[0xFF] < PHASE_BOUND
+    assert e[0x00] > 5 * e[0xFF]
+    assert e[0xFF] <= e[0xC1] <= e[0x81] * 1.05 and e[0x81] < e[0x00]
+
+
+def test_three_piece_step_mask_option(c1):
+    """Mask 255 is the default bit for bit; a mask selects launches that
+    graph replays follow; masks beyond eight bits are refused."""
+    s, p = c1
+    nw = p["nwalk"]
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, 6)
+    out = []
+    for mask, graphs in ((None, False), (0xFF, False), (0x81, False), (0x81, True)):
+        g = m.AfqmcSys(s, nw, "mixed_bf16", three_piece=4)
+        if mask is not None:
+            g.set_three_piece_steps(mask)
+        g.init_walkers()
+        g.stage_fields(x)
+        g.calculate_mixed_density_matrix()
+        if graphs:
+            g.use_graphs(True)
+        g.run(2, 3, 2, s.dt, energy=False)
+        out.append(g.W)
+        if mask == 0x81 and graphs:
+            with pytest.raises(nat.AfqmcError):
+                nat.check(nat.lib.afqmc_set_option(g.ctx, nat.OPT_THREE_PIECE_STEPS, 256.0), g.ctx)
+        g.close()
+    assert np.array_equal(out[0], out[1])
+    assert not np.array_equal(out[1], out[2])
+    assert np.array_equal(out[2], out[3])
