@@ -426,6 +426,12 @@ def main():
                       f"({r['peak_unit']}) = {r['frac']:.3f}, {r['ms_total']:.1f} ms", flush=True)
 
     extra = {}
+    if stats:
+        # the same kernel time under the reference driver's timer names
+        # (src/miniapps/miniafqmc_base.cpp:50-77); Total is the timed block
+        rt = m.reference_timers(stats)
+        extra["reference_timers_ms"] = {"Total": round(1e3 * T, 2), **{k_: round(ms, 2) for k_, (n_, ms) in rt.items()}}
+        print("[bench] " + ", ".join(f"{k_} {v_:.1f} ms" for k_, v_ in extra["reference_timers_ms"].items()), flush=True)
     if args.fp64 and args.mode != "fp64_dense":
         T64, eav64, r64 = run_mode("fp64_dense", True, state=res.get("state"))
         Wd64 = r64["Wd"]

@@ -21,6 +21,39 @@ def _dp(a):
     return a.view(np.float64).ctypes.data_as(_D) if a.dtype == np.complex128 else a.ctypes.data_as(_D)
 
 
+# The reference driver's timers (src/miniapps/miniafqmc_base.cpp:50-77, the
+# phases its step loop brackets at :248-350) and the library call sites whose
+# GPU time each one covers.  "Total" is the wall time of the run; launches at
+# upload and the tile autotuner's trial launches fall under no timer.
+REFERENCE_TIMERS = (
+    ("compact Mixed Density Matrix", ("gc_gemm", "det_restore")),
+    ("Mixed Density Matrix", ("gfull_gemm",)),
+    ("Bias Potential", ("vbias_gemm", "vbias_reduce", "csrmm_vbias", "csrmm_vbias_t")),
+    ("H-S Potential", ("vhs_gemm", "csrmm_vhs", "vhs_transpose")),
+    ("Sigma", ("fields", "set_substep")),
+    ("Propagation", ("prop_pw", "prop_taylor", "prop_ps", "prop_split_w", "prop_split_s", "prop_fused")),
+    ("Other", ("save_overlaps", "weights", "branch_select", "branch_gather")),
+    ("Overlap", ("ovlp_gemm", "gj_inverse")),
+    ("Orthgonalization", ("ortho_gram", "ortho_chol_inv", "ortho_sign", "ortho_apply", "ortho_flags",
+                          "lq_orthogonalize", "lq_fallback")),
+    ("Local Energy", ("energy_one", "energy_two", "energy_direct", "energy_exch_dot", "energy_pairsum",
+                      "energy_assemble", "energy_zero", "energy_trace", "energy_v_gemm", "energy_vx_gemm",
+                      "energy_y_gemm", "csrmm_vakbl", "g_transpose", "gc64_gemm", "convert", "g_split3")),
+)
+
+
+def reference_timers(stats):
+    """Per-call-site kernel statistics ({name: (launches, ms)}, as
+    AfqmcSys.kernel_stats returns) summed under the reference driver's timer
+    names: {timer: (launches, ms)}.  Call sites under no timer are left out."""
+    out = {}
+    for timer, sites in REFERENCE_TIMERS:
+        n = sum(stats[k][0] for k in sites if k in stats)
+        ms = sum(stats[k][1] for k in sites if k in stats)
+        out[timer] = (n, ms)
+    return out
+
+
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
                  use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1,

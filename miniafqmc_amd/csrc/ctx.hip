@@ -611,22 +611,22 @@ int gj_inverse_into(afqmc_ctx* c, zd* T, zd* Tinv, zd* det, long dq, long dr, in
   const long ms = (long)N * N;
   if (N <= 16)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<16, 4>), dim3(batch), dim3(64), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+      hipLaunchKernelGGL((gj_inverse_kernel<16, 8>), dim3(batch), dim3(128), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
                          flags, dc, ldet, lc);
     });
   if (N <= 32)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<32, 4>), dim3(batch), dim3(128), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+      hipLaunchKernelGGL((gj_inverse_kernel<32, 8>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
                          flags, dc, ldet, lc);
     });
   if (N <= 64)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<64, 4>), dim3(batch), dim3(256), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
+      hipLaunchKernelGGL((gj_inverse_kernel<64, 8>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N, ms,
                          flags, dc, ldet, lc);
     });
   if (N <= 128)
     return launch(c, "gj_inverse", [&]() {
-      hipLaunchKernelGGL((gj_inverse_kernel<128, 4>), dim3(batch), dim3(512), 0, c->stream, T, Tinv, det, dq, dr, N,
+      hipLaunchKernelGGL((gj_inverse_kernel<128, 8>), dim3(batch), dim3(1024), 0, c->stream, T, Tinv, det, dq, dr, N,
                          ms, flags, dc, ldet, lc);
     });
   if (N <= GJ_GMAX) {

@@ -17,7 +17,8 @@
 // registers (NMAX/CG complex values).  Rows >= N are padded with the
 // identity.  Per elimination step: holders of column k publish |X(i,k)|, every
 // wave finds the pivot row p, rows p and k and column k go through LDS, and
-// each thread updates its own entries.
+// each thread updates its own entries.  The signed pivots also give the
+// log-determinant (store_log_det).
 // batch b: input T + b*mstride; det written to det[(b/2)*dq + (b%2)*dr] and
 // to dcache[b] when dcache != null.
 // log det = sum_k log(pv_k), principal logs, imaginary part wrapped to
@@ -55,6 +56,34 @@ __device__ inline void store_log_det(const zd* pv, int N, bool singular, long b,
   }
 }
 
+// Wave argmax of (v, i) pairs, larger v first, then smaller i (an
+// associative, commutative choice, so any pairing order gives the same
+// pair): DPP moves within quads, half-rows and rows, then row broadcasts;
+// lane 63 holds the wave's pair, and i is returned in every lane.
+template <int CTRL, int RM> __device__ __forceinline__ void amx_step(double& bv, int& bi)
+{
+  const long long u = __double_as_longlong(bv);
+  const int lo = (int)u, hi = (int)(u >> 32);
+  const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xF, false);
+  const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(bi, bi, CTRL, RM, 0xF, false);
+  const double ov = __longlong_as_double((long long)(((unsigned long long)(unsigned)ohi << 32) | (unsigned)olo));
+  if (ov > bv || (ov == bv && oi < bi)) {
+    bv = ov;
+    bi = oi;
+  }
+}
+__device__ __forceinline__ int wave_argmax(double bv, int bi)
+{
+  amx_step<0xB1, 0xF>(bv, bi);   // quad_perm [1,0,3,2]
+  amx_step<0x4E, 0xF>(bv, bi);   // quad_perm [2,3,0,1]
+  amx_step<0x141, 0xF>(bv, bi);  // row_half_mirror
+  amx_step<0x140, 0xF>(bv, bi);  // row_mirror
+  amx_step<0x142, 0xA>(bv, bi);  // row_bcast15 into rows 1 and 3
+  amx_step<0x143, 0xC>(bv, bi);  // row_bcast31 into rows 2 and 3
+  return __builtin_amdgcn_readlane(bi, 63);
+}
+
 template <int NMAX, int CG>
 __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restrict__ T, zd* __restrict__ Tinv,
                                                                zd* __restrict__ det, long dq, long dr, int N,
@@ -80,84 +109,100 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
   }
   zd dt{1.0, 0.0};
   bool singular = false;
-  for (int k = 0; k < N; k++) {
-    const int tk = k / CG, ck = k % CG;
-    zd xk{0, 0};  // this thread's X(i,k) when it holds column k
-    if (cg == ck) {
+  // Columns are eliminated in blocks of CG: block tk holds columns CG*tk ..
+  // CG*tk + CG - 1, one per thread of a row, all in register slot tk.  For the
+  // CG steps of a block that slot is moved to xc, a register addressed
+  // statically, so only xc needs the column-k special case; the other slots
+  // take the plain update (slot tk's copy is refreshed from xc at the end).
+  const int nblk = (N + CG - 1) / CG;
+  for (int tk = 0; tk < nblk && !singular; tk++) {
+    zd xc{0, 0};
 #pragma unroll
-      for (int t = 0; t < NC; t++)
-        if (t == tk) xk = x[t];
-      red[i] = (i >= k) ? zcabs1(xk) : -1.0;
-    }
-    __syncthreads();
-    double bv = -2.0;
-    int bi = 0;
+    for (int t = 0; t < NC; t++)
+      if (t == tk) xc = x[t];
+    const int jc = cg + CG * tk;  // the column xc holds
 #pragma unroll
-    for (int q = lane; q < NMAX; q += 64) {
-      double v = red[q];
-      if (v > bv) { bv = v; bi = q; }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      double ov = __shfl_xor(bv, off);
-      int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    const int p = bi;
-    if (i == p) {
-#pragma unroll
-      for (int t = 0; t < NC; t++) prow[cg + CG * t] = x[t];
-    }
-    if (i == k && p != k) {
-#pragma unroll
-      for (int t = 0; t < NC; t++) oldk[cg + CG * t] = x[t];
-    }
-    if (cg == ck) colk[i] = xk;
-    if (cg == ck && i == p) pv[k] = (p != k) ? zd{-xk.re, -xk.im} : xk;  // signed pivot
-    if (tid == 0) perm[k] = p;
-    __syncthreads();
-    const zd piv = prow[k];
-    if (piv.re == 0.0 && piv.im == 0.0) {
-      singular = true;
-      dt = zd{0.0, 0.0};
-      break;
-    }
-    dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
-    const zd ipv = zdiv(zd{1.0, 0.0}, piv);
-    if (i == k) {
-      // new row k = pivot row / piv; column k holds 1/piv
-#pragma unroll
-      for (int t = 0; t < NC; t++) {
-        int j = cg + CG * t;
-        x[t] = (j == k) ? ipv : zmul(prow[j], ipv);
+    for (int ck = 0; ck < CG; ck++) {
+      const int k = CG * tk + ck;
+      if (k >= N) break;
+      const bool own = (cg == ck);  // xc holds column k
+      if (own) {
+        red[i] = (i >= k) ? zcabs1(xc) : -1.0;
+        colk[i] = xc;
       }
-    } else {
-      const bool swapped_in = (i == p);  // p != k here since i != k
-      if (swapped_in) {
+      __syncthreads();
+      double bv = -2.0;
+      int bi = 0;
 #pragma unroll
-        for (int t = 0; t < NC; t++) x[t] = oldk[cg + CG * t];
+      for (int q = lane; q < NMAX; q += 64) {
+        double v = red[q];
+        if (v > bv) { bv = v; bi = q; }
       }
-      const zd f = swapped_in ? colk[k] : colk[i];
-      const zd gq = zmul(zd{-f.re, -f.im}, ipv);  // -X(i,k)/piv
+      const int p = wave_argmax(bv, bi);
+      if (i == p) {
 #pragma unroll
-      for (int t = 0; t < NC; t++) {
-        const int j = cg + CG * t;
-        const zd pr = prow[j];
-        const bool kk = (j == k);
-        const double xr = kk ? 0.0 : x[t].re, xi = kk ? 0.0 : x[t].im;
-        const double pr_r = kk ? 1.0 : pr.re, pr_i = kk ? 0.0 : pr.im;
-        x[t].re = xr + gq.re * pr_r - gq.im * pr_i;
-        x[t].im = xi + gq.re * pr_i + gq.im * pr_r;
+        for (int t = 0; t < NC; t++) prow[cg + CG * t] = x[t];
+        prow[jc] = xc;  // after slot tk's stale copy: the later store wins
+        if (own) pv[k] = (p != k) ? zd{-xc.re, -xc.im} : xc;  // signed pivot
+      }
+      if (i == k && p != k) {
+#pragma unroll
+        for (int t = 0; t < NC; t++) oldk[cg + CG * t] = x[t];
+        oldk[jc] = xc;
+      }
+      if (tid == 0) perm[k] = p;
+      __syncthreads();
+      const zd piv = prow[k];
+      if (piv.re == 0.0 && piv.im == 0.0) {
+        singular = true;
+        dt = zd{0.0, 0.0};
+        break;
+      }
+      dt = zmul(dt, (p != k) ? zd{-piv.re, -piv.im} : piv);
+      const zd ipv = zdiv(zd{1.0, 0.0}, piv);
+      if (i == k) {
+        // new row k = pivot row / piv; column k holds 1/piv
+#pragma unroll
+        for (int t = 0; t < NC; t++) x[t] = zmul(prow[cg + CG * t], ipv);
+        xc = own ? ipv : zmul(prow[jc], ipv);
+      } else {
+        const bool swapped_in = (i == p);  // p != k here since i != k
+        if (swapped_in) {
+#pragma unroll
+          for (int t = 0; t < NC; t++) x[t] = oldk[cg + CG * t];
+          xc = oldk[jc];
+        }
+        const zd f = swapped_in ? colk[k] : colk[i];
+        const zd gq = zmul(zd{-f.re, -f.im}, ipv);  // -X(i,k)/piv
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+          const zd pr = prow[cg + CG * t];
+          const double xr = x[t].re, xi = x[t].im;
+          x[t].re = xr + gq.re * pr.re - gq.im * pr.im;
+          x[t].im = xi + gq.re * pr.im + gq.im * pr.re;
+        }
+        // column k: X(i,k) = -X(i,k)/piv, i.e. 0 + gq * 1
+        const zd pc = prow[jc];
+        const double xr = own ? 0.0 : xc.re, xi = own ? 0.0 : xc.im;
+        const double pr_r = own ? 1.0 : pc.re, pr_i = own ? 0.0 : pc.im;
+        xc.re = xr + gq.re * pr_r - gq.im * pr_i;
+        xc.im = xi + gq.re * pr_i + gq.im * pr_r;
       }
     }
+#pragma unroll
+    for (int t = 0; t < NC; t++)
+      if (t == tk) x[t] = xc;
   }
   if (tid == 0) {
     if (singular && info) atomicAdd(info, 1);
     det[(b / 2) * dq + (b % 2) * dr] = dt;
     if (dcache) dcache[b] = dt;
   }
-  store_log_det(pv, N, singular, b, ldet, lcache);
-  if (!Tinv) return;
+  // the log-determinant last, when the matrix no longer occupies registers
+  if (!Tinv) {
+    store_log_det(pv, N, singular, b, ldet, lcache);
+    return;
+  }
   if (singular) {
     // leave a defined value: zeros
     __syncthreads();
@@ -166,6 +211,7 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
       int j = cg + CG * t;
       if (i < N && j < N) Tinv[b * mstride + (long)j * N + i] = zd{0, 0};
     }
+    store_log_det(pv, N, singular, b, ldet, lcache);
     return;
   }
   // undo the row interchanges as column interchanges in reverse order
@@ -188,6 +234,7 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
     int j = cg + CG * t;
     if (i < N && j < N) Tinv[b * mstride + (long)perm[j] * N + i] = x[t];
   }
+  store_log_det(pv, N, singular, b, ldet, lcache);
 }
 
 // The same Gauss-Jordan elimination for N too large for the registers of one

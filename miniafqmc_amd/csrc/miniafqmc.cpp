@@ -25,6 +25,30 @@
 
 namespace {
 
+// The reference driver's timers (miniafqmc_base.cpp:50-77) and the library
+// call sites whose GPU time each one covers, for the -v report.  Launches at
+// upload fall under none.
+struct TimerSites {
+  const char* timer;
+  const char* sites[17];
+};
+const TimerSites kTimers[] = {
+    {"compact Mixed Density Matrix", {"gc_gemm", "det_restore"}},
+    {"Mixed Density Matrix", {"gfull_gemm"}},
+    {"Bias Potential", {"vbias_gemm", "vbias_reduce", "csrmm_vbias", "csrmm_vbias_t"}},
+    {"H-S Potential", {"vhs_gemm", "csrmm_vhs", "vhs_transpose"}},
+    {"Sigma", {"fields", "set_substep"}},
+    {"Propagation", {"prop_pw", "prop_taylor", "prop_ps", "prop_split_w", "prop_split_s", "prop_fused"}},
+    {"Other", {"save_overlaps", "weights", "branch_select", "branch_gather"}},
+    {"Overlap", {"ovlp_gemm", "gj_inverse"}},
+    {"Orthgonalization",
+     {"ortho_gram", "ortho_chol_inv", "ortho_sign", "ortho_apply", "ortho_flags", "lq_orthogonalize", "lq_fallback"}},
+    {"Local Energy",
+     {"energy_one", "energy_two", "energy_direct", "energy_exch_dot", "energy_pairsum", "energy_assemble",
+      "energy_zero", "energy_trace", "energy_v_gemm", "energy_vx_gemm", "energy_y_gemm", "csrmm_vakbl", "g_transpose",
+      "gc64_gemm", "convert", "g_split3"}},
+};
+
 void print_help()
 {
   std::printf("miniafqmc - AFQMC walker step on one MI355X\n\n");
@@ -36,7 +60,7 @@ void print_help()
   std::printf("-f                Input file name (afqmc.h5 format; read through libafqmc_h5.so)\n");
   std::printf("-t                yes|no: half-rotated transposed Cholesky matrix for the bias potential\n");
   std::printf("                  ('no': full G and T(Spvn), fp64_csr mode only)\n");
-  std::printf("-v                Verbose output (per-kernel timers)\n");
+  std::printf("-v                Verbose output (GPU time under the reference's timer names, then per kernel)\n");
   std::printf("--dt D            time step (default 0.005)\n");
   std::printf("--mode M          fp64_csr | fp64_dense | mixed_fp32 | mixed_bf16 (default fp64_csr)\n");
   std::printf("--system S        N16_M64 | N64_M256 | N128_M512 | MxNxK (synthetic, when no -f; default N16_M64)\n");
@@ -298,6 +322,22 @@ int main(int argc, char** argv)
     double ms[cap];
     int cnt = 0;
     CHECK(afqmc_kernel_stats(ctx, cap, names, n, ms, &cnt));
+    // the reference's timer report: the same GPU time under its phase names
+    std::printf("\n%-30s %12s %10s\n", "Timer", "GPU time s", "launches");
+    std::printf("%-30s %12.6f %10s\n", "Total (wall)", total, "");
+    for (const TimerSites& t : kTimers) {
+      double sum = 0.0;
+      long long calls = 0;
+      for (const char* site : t.sites) {
+        if (!site) break;
+        for (int i = 0; i < cnt && i < cap; i++)
+          if (std::strcmp(names[i], site) == 0) {
+            sum += ms[i];
+            calls += n[i];
+          }
+      }
+      std::printf("%-30s %12.6f %10lld\n", t.timer, sum * 1e-3, calls);
+    }
     std::printf("\n%-24s %10s %14s\n", "kernel", "launches", "total ms");
     for (int i = 0; i < cnt && i < cap; i++) std::printf("%-24s %10lld %14.3f\n", names[i], n[i], ms[i]);
   }
