@@ -236,6 +236,9 @@ def main():
     ap.add_argument("--three-steps", type=lambda v: int(v, 0), default=0x81,
                     help="with the propagator in three pieces: a mask of its eight products that take three "
                          "pieces (bit 0 P W, bits 1-6 the Horner steps U_6 .. U_1, bit 7 P U)")
+    ap.add_argument("--one-steps", type=lambda v: int(v, 0), default=0x0E,
+                    help="with the propagator in three pieces: a mask of the Horner steps (bits 1-6: U_6 .. U_1) "
+                         "whose products take one piece (the hi bf16 words)")
     ap.add_argument("--variant", type=int, default=1,
                     help="split-BF16 mode: also time the other piece count (two pieces if --three-piece is set, "
                          "else all three-piece) from the same walkers and report it beside the value")
@@ -323,6 +326,7 @@ def main():
         g.set_direct_lds(args.direct_lds)
         if mode == "mixed_bf16" and three & 4:
             g.set_three_piece_steps(args.three_steps)
+            g.set_one_piece_steps(args.one_steps)
         # the auxiliary fields are drawn on the device inside each substep
         # (the reference's generate_normal in the step loop), the same
         # counter-based draws the oracle is given on the host
@@ -539,6 +543,8 @@ def main():
                    "graphs": bool(args.graphs), "direct_lds": args.direct_lds,
                    "three_piece": args.three_piece if args.mode == "mixed_bf16" else 0,
                    "three_piece_steps": args.three_steps if args.mode == "mixed_bf16" and args.three_piece & 4
+                   else None,
+                   "one_piece_steps": args.one_steps if args.mode == "mixed_bf16" and args.three_piece & 4
                    else None,
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},

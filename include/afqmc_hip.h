@@ -149,7 +149,15 @@ typedef enum {
    * the Horner steps U_6 .. U_1, bit 7 P U); the others take two pieces of
    * the same operands.  Default 255 (all).  May be changed at any time.  No
    * counterpart in the reference. */
-  AFQMC_OPT_THREE_PIECE_STEPS = 9
+  AFQMC_OPT_THREE_PIECE_STEPS = 9,
+  /* with bit 4 of AFQMC_OPT_THREE_PIECE: a mask of the Horner steps (bits
+   * 1-6, as above; bits 0 and 7 are refused) whose products take one piece,
+   * the hi bf16 words of the split operands (8 significant bits, one MFMA
+   * product), unless AFQMC_OPT_THREE_PIECE_STEPS gives them three.  The
+   * innermost steps are scaled by V^k/k!, so their rounding barely reaches
+   * the walkers.  Default 0.  May be changed at any time.  No counterpart in
+   * the reference. */
+  AFQMC_OPT_ONE_PIECE_STEPS = 10
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

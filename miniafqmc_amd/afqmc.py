@@ -216,6 +216,12 @@ class AfqmcSys:
         others take two pieces.  255 (the default) is all."""
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_THREE_PIECE_STEPS, float(mask)))
 
+    def set_one_piece_steps(self, mask):
+        """With the propagator in three pieces (bit 4 of `three_piece`): a
+        mask of the Horner steps (bits 1-6: U_6 .. U_1) whose products take
+        one piece, the hi bf16 words of the split operands."""
+        self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ONE_PIECE_STEPS, float(mask)))
+
     def set_exchange_chunk(self, cols):
         """Dense exchange: accumulate the exchange GEMM over chunks of `cols`
         columns, adding the chunks' energies in FP64 (0: one pass)."""

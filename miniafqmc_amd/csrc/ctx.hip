@@ -153,9 +153,11 @@ struct afqmc_ctx {
   uint32_t* tp_Ta = nullptr;   // [nw][2][N][M] (propagator terms)
   uint32_t* tp_Tb = nullptr;
   // with bit 4 of `three`: which of the propagator's eight products take
-  // three pieces (bit 0 P W, bit 7-k the Horner step k = 6..1, bit 7 P U);
-  // the others take two pieces of the same split operands
+  // three pieces (bit 0 P W, bit 7-k the Horner step k = 6..1, bit 7 P U)
+  // and which Horner steps take one (the hi words of the same split
+  // operands); the others take two pieces
   int three_steps = 0xFF;
+  int one_steps = 0;
   bool allow_vx = true;
   zd* Gtmp = nullptr;     // FP64 G [nw][2NM] for the energy (CSR: transposed; mixed: FP64 copy)
   void* vws = nullptr;    // split-k partials of the bias contraction
@@ -486,6 +488,15 @@ template <class OT, int NP = 2> int zgemm_ring(afqmc_ctx* c, const char* name, c
   return ring_run<OT, NP>(c, name, a, batch, best);
 }
 
+// one-piece products of split operands (CP_BF16X1, the hi words only): the
+// staged-ring kernel only, whatever AFQMC_OPT_DIRECT_LDS says
+bool ring1_ok(const ZGemmArgs& a)
+{
+  return a.sak == 1 && a.sbk == 1 && a.k % 16 == 0 && a.sam % 2 == 0 && a.sa1 % 2 == 0 && a.sa2 % 2 == 0 &&
+         a.sbn % 2 == 0 && a.sb1 % 2 == 0 && a.sb2 % 2 == 0 && ((uintptr_t)a.A & 15) == 0 &&
+         ((uintptr_t)a.B & 15) == 0 && !a.conjb;
+}
+
 // three-piece split-BF16 operands (CP_BF16X6): the staged-ring kernel only
 bool ring3_ok(const ZGemmArgs& a)
 {
@@ -504,6 +515,11 @@ int zgemm(afqmc_ctx* c, const char* name, ZGemmArgs a, int batch)
                   "three-piece products take split operands with k contiguous");
     if (!ring3_ok(a)) return fail(c, AFQMC_ERR_UNSUPPORTED, std::string(name) + ": three-piece operand layout");
     return zgemm_ring<OT, 3>(c, name, a, batch);
+  } else if constexpr (CP == CP_BF16X1) {
+    static_assert(std::is_same<AT, bfs>::value && std::is_same<BT, bfs>::value && AK && BK,
+                  "one-piece products take split operands with k contiguous");
+    if (!ring1_ok(a)) return fail(c, AFQMC_ERR_UNSUPPORTED, std::string(name) + ": one-piece operand layout");
+    return zgemm_ring<OT, 1>(c, name, a, batch);
   } else {
   if constexpr (CP == CP_BF16X3 && std::is_same<AT, bfs>::value && std::is_same<BT, bfs>::value && AK && BK) {
     if (ring_ok(c, a)) return zgemm_ring<OT>(c, name, a, batch);
@@ -1087,6 +1103,7 @@ int propagate_three(afqmc_ctx* c)
   const dim3 tgrid((unsigned)((N + 31) / 32), (unsigned)((M + 31) / 32), (unsigned)(2 * nw));
   // product j in launch order: 0 P W, 7-k the Horner step k, 7 P U
   auto three = [&](int j) { return ((c->three_steps >> j) & 1) != 0; };
+  auto one = [&](int j) { return !three(j) && ((c->one_steps >> j) & 1) != 0; };
   zb* Ta = (zb*)c->Ta;
   zb* Tb = (zb*)c->Tb;
   RET(launch(c, "prop_split_w", [&]() {
@@ -1150,6 +1167,7 @@ int propagate_three(afqmc_ctx* c)
     a.sfn = 1;
     a.cf = 1;
     if (three(7 - k)) RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
+    else if (one(7 - k)) RET((zgemm<CP_BF16X1, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
     else RET((zgemm<CP_BF16X3, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
     prev = bufs[k & 1];
     prev3 = bufs3[k & 1];
@@ -1965,6 +1983,13 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
       return fail(c, AFQMC_ERR_ARG, "three-piece propagator step mask is 0..255");
     drop_graphs(c);
     c->three_steps = (int)value;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_ONE_PIECE_STEPS) {
+    if (value < 0 || value > 255 || value != (int)value || ((int)value & 0x81))
+      return fail(c, AFQMC_ERR_ARG, "one-piece step mask takes Horner steps only (bits 1-6)");
+    drop_graphs(c);
+    c->one_steps = (int)value;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_TILE) {

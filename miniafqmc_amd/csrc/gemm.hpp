@@ -25,7 +25,7 @@
 #include "common.hpp"
 #include <type_traits>
 
-enum { CP_F64 = 0, CP_F32 = 1, CP_BF16X3 = 2, CP_BF16X6 = 3 };
+enum { CP_F64 = 0, CP_F32 = 1, CP_BF16X3 = 2, CP_BF16X6 = 3, CP_BF16X1 = 4 };
 
 struct ZGemmArgs {
   const void* A;

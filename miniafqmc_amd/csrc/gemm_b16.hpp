@@ -30,6 +30,9 @@
 // of 64 bytes, chunk c of row R at c ^ ((R >> 2) & 3), conflict-free for the
 // 16 rows of a fragment read), and the six products h*h, h*m, m*h, h*l, l*h,
 // m*m accumulate in that order.
+//
+// NP = 1 (one piece, 8 significant bits): the same two-piece tiles are staged
+// and only the hi words enter the MFMAs, h*h alone.
 #pragma once
 #include "gemm.hpp"
 
@@ -67,8 +70,8 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
   constexpr int LPT = AL + BL + AL3 + BL3;
   // one stage in uint2 units: the two-piece tiles, then the third pieces
   constexpr int STG = (BM + BN) * KT + (NP == 3 ? (BM + BN) * KT / 2 : 0);
-  static_assert(NP == 2 || NP == 3, "two or three pieces");
-  static_assert(NP == 2 || ((BM * C3) % NT == 0 && (BN * C3) % NT == 0), "third-piece tiles must split evenly");
+  static_assert(NP >= 1 && NP <= 3, "one, two or three pieces");
+  static_assert(NP != 3 || ((BM * C3) % NT == 0 && (BN * C3) % NT == 0), "third-piece tiles must split evenly");
   static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile must be a multiple of 16");
   static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tiles must split evenly over the threads");
   static_assert(STAGES >= 2 && STAGES <= 4, "2 to 4 stages");
@@ -196,7 +199,7 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
       f.ah[i] = make_uint4(v0.x, v0.z, v1.x, v1.z);
       f.al[i] = make_uint4(v0.y, v0.w, v1.y, v1.w);
     }
-    if constexpr (NP == 2) {
+    if constexpr (NP != 3) {
 #pragma unroll
       for (int j = 0; j < TN; j++) {
         f.b0[j] = t4[(brow + 16 * j) * 8 + c0];
@@ -220,9 +223,14 @@ __global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
 #pragma unroll
     for (int j = 0; j < TN; j++) {
       const uint4* t4 = (const uint4*)st;
-      const uint4 v0 = NP == 2 ? f.b0[j % TNF] : t4[(brow + 16 * j) * 8 + c0];
-      const uint4 v1 = NP == 2 ? f.b1[j % TNF] : t4[(brow + 16 * j) * 8 + c1];
-      if constexpr (NP == 2) {
+      const uint4 v0 = NP != 3 ? f.b0[j % TNF] : t4[(brow + 16 * j) * 8 + c0];
+      const uint4 v1 = NP != 3 ? f.b1[j % TNF] : t4[(brow + 16 * j) * 8 + c1];
+      if constexpr (NP == 1) {
+        const uint4 rh = make_uint4(ng(v0.x), ng(v0.z), ng(v1.x), ng(v1.z));
+        const uint4 ih = make_uint4(rt(v0.x), rt(v0.z), rt(v1.x), rt(v1.z));
+#pragma unroll
+        for (int i = 0; i < TM; i++) mm(i, j, f.ah[i], rh, ih);
+      } else if constexpr (NP == 2) {
         const uint4 rh = make_uint4(ng(v0.x), ng(v0.z), ng(v1.x), ng(v1.z));
         const uint4 rl = make_uint4(ng(v0.y), ng(v0.w), ng(v1.y), ng(v1.w));
         const uint4 ih = make_uint4(rt(v0.x), rt(v0.z), rt(v1.x), rt(v1.z));

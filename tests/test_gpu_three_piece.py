@@ -202,3 +202,51 @@ def test_three_piece_step_mask_option(c1):
     assert np.array_equal(out[0], out[1])
     assert not np.array_equal(out[1], out[2])
     assert np.array_equal(out[2], out[3])
+
+
+def test_one_piece_horner_steps(c1):
+    """The innermost Horner steps in one piece (the hi bf16 words): their
+    terms carry V^k/k!, so the propagated walkers stay at the two-piece
+    level against the oracle, while all six steps in one piece do not;
+    graph replays follow the mask, and P W / P U cannot take one piece."""
+    s, p = c1
+    e = {}
+    for one in (0x00, 0x0E, 0x7E):
+        nw = p["nwalk"]
+        g = m.AfqmcSys(s, nw, "mixed_bf16", three_piece=4)
+        g.set_three_piece_steps(0x81)
+        g.set_one_piece_steps(one)
+        W = perturbed_walkers(s, nw, seed=11)
+        spT = orc.halfrotate(s, cutoff=-1.0)
+        Wd = np.zeros((nw, 8), complex)
+        Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+        Xo, _ = orc.fields(orc.get_vbias(spT, Go), m.fields(p["seed"], s.nchol, nw, 0))
+        vhso = orc.get_vhs(s.spvn, Xo)
+        g.set(nat.BUF_W, W)
+        g.set(nat.BUF_VHS, vhso)
+        g.propagate()
+        e[one] = rel(g.W, orc.propagate(s, W, vhso))
+        if one == 0x0E:
+            for bad in (0x01, 0x80, 256):
+                with pytest.raises(nat.AfqmcError):
+                    nat.check(nat.lib.afqmc_set_option(g.ctx, nat.OPT_ONE_PIECE_STEPS, float(bad)), g.ctx)
+        g.close()
+    say("one-piece Horner steps: " + ", ".join(f"0x{k:02x} {v:.2e}" for k, v in e.items()))
+    assert e[0x0E] < 2 * e[0x00] + 1e-7
+    assert e[0x7E] > 5 * e[0x0E]
+    nw = p["nwalk"]
+    x = m.fields_block(p["seed"], s.nchol, nw, 0, 6)
+    out = []
+    for graphs in (False, True):
+        g = m.AfqmcSys(s, nw, "mixed_bf16", three_piece=4)
+        g.set_three_piece_steps(0x81)
+        g.set_one_piece_steps(0x0E)
+        g.init_walkers()
+        g.stage_fields(x)
+        g.calculate_mixed_density_matrix()
+        if graphs:
+            g.use_graphs(True)
+        g.run(2, 3, 2, s.dt, energy=False)
+        out.append(g.W)
+        g.close()
+    assert np.array_equal(out[0], out[1])
