@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--three-steps", type=lambda v: int(v, 0), default=0x81,
                     help="with the propagator in three pieces: a mask of its eight products that take three "
                          "pieces (bit 0 P W, bits 1-6 the Horner steps U_6 .. U_1, bit 7 P U)")
-    ap.add_argument("--one-steps", type=lambda v: int(v, 0), default=0x0E,
+    ap.add_argument("--one-steps", type=lambda v: int(v, 0), default=0x06,
                     help="with the propagator in three pieces: a mask of the Horner steps (bits 1-6: U_6 .. U_1) "
                          "whose products take one piece (the hi bf16 words)")
     ap.add_argument("--variant", type=int, default=1,

@@ -157,7 +157,13 @@ typedef enum {
    * innermost steps are scaled by V^k/k!, so their rounding barely reaches
    * the walkers.  Default 0.  May be changed at any time.  No counterpart in
    * the reference. */
-  AFQMC_OPT_ONE_PIECE_STEPS = 10
+  AFQMC_OPT_ONE_PIECE_STEPS = 10,
+  /* AFQMC_FP64_CSR with the transposed bias, set before afqmc_upload_system:
+   * 1 (default) keeps a dense copy of the half-rotated SpvnT when at least
+   * half of its entries are stored (those under the cutoff are zero) and
+   * runs the bias product on FP64 MFMA; 0 always runs the CSR product.  No
+   * counterpart in the reference (its csrmm is the only form). */
+  AFQMC_OPT_CSR_DENSE = 11
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

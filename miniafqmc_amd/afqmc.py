@@ -57,7 +57,7 @@ def reference_timers(stats):
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
                  use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1,
-                 transposed: bool = True, three_piece: int = 0):
+                 transposed: bool = True, three_piece: int = 0, csr_dense: bool = True):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
@@ -67,6 +67,9 @@ class AfqmcSys:
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_DENSE_EXCHANGE, 1.0 if dense_exchange else 0.0))
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_ENERGY_PRECISION, float(energy_precision)))
         self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_TRANSPOSED, 1.0 if transposed else 0.0))
+        if mode == "fp64_csr":
+            # a half-full (or fuller) SpvnT also kept dense for the bias product
+            self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_CSR_DENSE, 1.0 if csr_dense else 0.0))
         if three_piece:
             # split-BF16 mode: contractions in three-piece split-BF16 (1 bias, 2 vHS, 4 propagator)
             self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_THREE_PIECE, float(three_piece)))

@@ -68,6 +68,10 @@ struct afqmc_ctx {
   int* spt_col = nullptr;
   zd* spt_val = nullptr;
   long long spt_nnz = 0;
+  // FP64_CSR: SpvnT as a dense row-major [K][2NM] copy when at least half
+  // of it is stored (AFQMC_OPT_CSR_DENSE), for the bias product on MFMA
+  zd* sptd = nullptr;
+  bool csr_dense = true;
   // non-transposed bias (miniafqmc_base.cpp:267-276): T(Spvn) as CSR [K][M*M]
   // and the full density matrix G [2*M*M][nw]
   bool transposed = true;
@@ -913,10 +917,54 @@ int do_vbias_full(afqmc_ctx* c)
   return AFQMC_OK;
 }
 
+// FP64_CSR bias product with SpvnT kept dense: vbias[K][nw] = SpvnT[K][2NM]
+// G[2NM][nw] (walker-last) on FP64 MFMA, the long sum split into S chunks
+// (a batched GEMM into a workspace, then summed in FP64 in chunk order).
+int dense_csr_vbias(afqmc_ctx* c)
+{
+  const long NAK = 2L * c->N * c->M, nw = c->nw, K = c->K;
+  const long tiles = ((K + 63) / 64) * ((nw + 31) / 32);
+  int S = 1;
+  while (S < 32 && S * tiles < 1024 && NAK % (2 * S) == 0 && (NAK / (2 * S)) % 64 == 0) S *= 2;
+  const long chunk = NAK / S;
+  ZGemmArgs a = gargs();
+  a.A = c->sptd;
+  a.sam = NAK;
+  a.sak = 1;
+  a.B = c->G;
+  a.sbn = 1;
+  a.sbk = nw;
+  a.m = (int)K;
+  a.n = (int)nw;
+  a.k = (int)chunk;
+  a.scm = nw;
+  a.scn = 1;
+  if (S == 1) {
+    a.C = c->vbias;
+    return zgemm<CP_F64, double, double, double, true, false>(c, "vbias_gemm", a, 1);
+  }
+  const size_t need = sizeof(zd) * S * K * nw;
+  if (c->vws_bytes < need) {
+    dfree(c, c->vws);
+    c->vws_bytes = need;
+    RET(dalloc_bytes(c, &c->vws, need));
+  }
+  a.sa1 = chunk;  // batch b: columns b*chunk .. of SpvnT, rows b*chunk .. of G
+  a.sb1 = chunk * nw;
+  a.C = c->vws;
+  a.sc1 = K * nw;
+  RET((zgemm<CP_F64, double, double, double, true, false>(c, "vbias_gemm", a, S)));
+  return launch(c, "vbias_reduce", [&]() {
+    hipLaunchKernelGGL((zsum_slices_kernel<zd>), dim3(nblk(nw * K, 256)), dim3(256), 0, c->stream, (const zd*)c->vws,
+                       c->vbias, nw * K, S);
+  });
+}
+
 int do_vbias(afqmc_ctx* c)
 {
   const int nw = c->nw, K = c->K, NAK = 2 * c->N * c->M;
   if (c->mode == AFQMC_FP64_CSR) {
+    if (c->sptd) return dense_csr_vbias(c);
     zd one{1, 0}, zero{0, 0};
     return csrmm_n(c, "csrmm_vbias", K, NAK, c->spt_nnz, c->spt_rp, c->spt_col, c->spt_val, (const zd*)c->G,
                    (long)nw, c->vbias, (long)nw, one, zero, true);
@@ -1821,8 +1869,9 @@ int capture_prep(afqmc_ctx* c, int ortho)
     RET(dalloc(c, &c->ofail, nb));
   }
   if (!c->transposed && !c->Gfull) RET(dalloc(c, &c->Gfull, 2L * c->M * c->M * c->nw));
-  if (c->mode != AFQMC_FP64_CSR) {
-    // the bias contraction's split-k workspace at its largest
+  if (c->mode != AFQMC_FP64_CSR || c->sptd) {
+    // the bias contraction's split-k workspace at its largest (at most 32
+    // chunks, also for the dense copy of SpvnT)
     const size_t need = sizeof(zd) * 32 * (size_t)c->nw * c->K;
     if (c->vws_bytes < need) {
       dfree(c, c->vws);
@@ -1950,6 +1999,11 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
       if (value < 0 || value > 3 || value != (int)value) return fail(c, AFQMC_ERR_ARG, "energy precision level is 0..3");
       c->energy_level = (int)value;
     }
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_CSR_DENSE) {
+    if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
+    c->csr_dense = value != 0.0;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_TRANSPOSED) {
@@ -2237,6 +2291,17 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     }));
     HIPCHK(c, hipStreamSynchronize(st));
     dfree(c, cnt);
+    if (c->csr_dense && c->transposed && 2 * tot >= K * 2 * NM) {
+      // a CSR product over rows that are at least half full is a dense
+      // product: keep SpvnT dense as well (entries under the cutoff are 0)
+      RET(dalloc(c, &c->sptd, K * 2 * NM));
+      HIPCHK(c, hipMemsetAsync(c->sptd, 0, sizeof(zd) * K * 2 * NM, st));
+      RET(launch(c, "csr_densify", [&]() {
+        hipLaunchKernelGGL(csr_densify_rows_kernel, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, st, (int)K,
+                           c->spt_rp, c->spt_col, c->spt_val, c->sptd, 2 * NM);
+      }));
+      HIPCHK(c, hipStreamSynchronize(st));
+    }
     dfree(c, Ld64);
     if (vakbl && vakbl->nnz > 0) {
       const long vn = vakbl->nnz, vr = vakbl->nrows;

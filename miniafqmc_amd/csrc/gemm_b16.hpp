@@ -56,8 +56,10 @@ __device__ __forceinline__ int swz3(int row) { return (row >> 2) & 3; }
 // output).  Launch conditions (checked by the host): A and B split-BF16, unit
 // k stride, rows and batch offsets 16-byte aligned, k % 16 == 0, no conj(B);
 // NP = 3 also third-piece rows 16-byte aligned.
-template <class OT, int BM, int BN, int WM, int WN, int STAGES, int NP = 2>
-__global__ __launch_bounds__(64 * WM * WN) void zgemm_b16dma_kernel(ZGemmArgs p)
+// WPE: waves per SIMD the register allocation must allow (0: no bound
+// beyond the workgroup size).
+template <class OT, int BM, int BN, int WM, int WN, int STAGES, int NP = 2, int WPE = 0>
+__global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmArgs p)
 {
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;  // rows, complex columns per wave

@@ -1356,13 +1356,14 @@ __global__ __launch_bounds__(NMAX* CG) void chol_inv_kernel(const zd* __restrict
         x[t] = (j < k) ? zd{0, 0} : ((j == k) ? zd{r, 0.0} : zd{x[t].re * rinv, x[t].im * rinv});
       }
     } else if (i > k && i < N) {
-      // conj(R_ki) R_kj = conj(S_ki) S_kj / d
+      // conj(R_ki) R_kj = conj(S_ki) S_kj / d, on every column: the entries
+      // left of the diagonal take garbage that is never read and is zeroed
+      // before the inverse (padded columns of row k are zero)
       const zd ski = prow[st][i];
       const zd f{ski.re / d, -ski.im / d};
 #pragma unroll
       for (int t = 0; t < NC; t++) {
-        const int j = cg + CG * t;
-        if (j >= i && j < N) x[t] = zsub(x[t], zmul(f, prow[st][j]));
+        x[t] = zsub(x[t], zmul(f, prow[st][cg + CG * t]));
         if ((t & 7) == 7) asm volatile("" ::: "memory");  // bound the LDS loads in flight
       }
     }
@@ -1374,42 +1375,52 @@ __global__ __launch_bounds__(NMAX* CG) void chol_inv_kernel(const zd* __restrict
     return;
   }
   // rows i < N now hold R (upper); rows >= N the identity.  In-place
-  // Gauss-Jordan inverse, pivots on the diagonal.
+  // Gauss-Jordan inverse, pivots on the diagonal, the columns in blocks of CG
+  // as gj_inverse_kernel: slot tk of block tk is held in xc, addressed
+  // statically, for the block's CG steps.
 #pragma unroll
   for (int t = 0; t < NC; t++) {
     const int j = cg + CG * t;
     if (j < i) x[t] = zd{0, 0};
   }
-  for (int k = 0; k < N; k++) {
-    const int tk = k / CG, ck = k % CG, st = k & 1;
-    if (i == k) {
+  const int nblk = (N + CG - 1) / CG;
+  for (int tk = 0; tk < nblk; tk++) {
+    zd xc{0, 0};
 #pragma unroll
-      for (int t = 0; t < NC; t++) prow[st][cg + CG * t] = x[t];
-    }
-    if (cg == ck) {
+    for (int t = 0; t < NC; t++)
+      if (t == tk) xc = x[t];
+    const int jc = cg + CG * tk;
 #pragma unroll
-      for (int t = 0; t < NC; t++)
-        if (t == tk) colk[st][i] = x[t];
-    }
-    __syncthreads();
-    const double ipv = 1.0 / prow[st][k].re;  // R_kk is real
-    if (i == k) {
+    for (int ck = 0; ck < CG; ck++) {
+      const int k = CG * tk + ck, st = k & 1;
+      if (k >= N) break;
+      const bool own = (cg == ck);
+      if (i == k) {
 #pragma unroll
-      for (int t = 0; t < NC; t++) {
-        const int j = cg + CG * t;
-        x[t] = (j == k) ? zd{ipv, 0.0} : zd{x[t].re * ipv, x[t].im * ipv};
+        for (int t = 0; t < NC; t++) prow[st][cg + CG * t] = x[t];
+        prow[st][jc] = xc;  // after slot tk's stale copy: the later store wins
       }
-    } else if (i < k) {
-      const zd g = colk[st][i];
-      const zd gq{-g.re * ipv, -g.im * ipv};
+      if (own) colk[st][i] = xc;
+      __syncthreads();
+      const double ipv = 1.0 / prow[st][k].re;  // R_kk is real
+      if (i == k) {
 #pragma unroll
-      for (int t = 0; t < NC; t++) {
-        const int j = cg + CG * t;
-        if (j == k) x[t] = gq;
-        else x[t] = zadd(x[t], zmul(gq, prow[st][j]));
-        if ((t & 7) == 7) asm volatile("" ::: "memory");
+        for (int t = 0; t < NC; t++) x[t] = zd{x[t].re * ipv, x[t].im * ipv};
+        xc = own ? zd{ipv, 0.0} : zd{xc.re * ipv, xc.im * ipv};
+      } else if (i < k) {
+        const zd g = colk[st][i];
+        const zd gq{-g.re * ipv, -g.im * ipv};
+#pragma unroll
+        for (int t = 0; t < NC; t++) {
+          x[t] = zadd(x[t], zmul(gq, prow[st][cg + CG * t]));
+          if ((t & 7) == 7) asm volatile("" ::: "memory");
+        }
+        xc = own ? gq : zadd(xc, zmul(gq, prow[st][jc]));
       }
     }
+#pragma unroll
+    for (int t = 0; t < NC; t++)
+      if (t == tk) x[t] = xc;
   }
   zd* Ub = U + b * (long)N * N;
 #pragma unroll
@@ -1590,6 +1601,17 @@ __global__ void csr_densify_kernel(int nrows, const int* __restrict__ rowptr, co
   if (r >= nrows) return;
   int p0 = rowptr[0];
   for (int p = rowptr[r] - p0 + lane; p < rowptr[r + 1] - p0; p += 64) Ld[(long)col[p] * ldn + r] = val[p];
+}
+
+// CSR rows into a dense row-major matrix D[r][col] (zeroed beforehand;
+// rowptr starts at 0, no repeated columns within a row)
+__global__ void csr_densify_rows_kernel(int nrows, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                        const zd* __restrict__ val, zd* __restrict__ D, long ld)
+{
+  const int r = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nrows) return;
+  for (int p = rowptr[r] + lane; p < rowptr[r + 1]; p += 64) D[(long)r * ld + col[p]] = val[p];
 }
 
 // per-row count of |Lt(n, j)| > cutoff, Lt spin-major [2][K][NM]: row n of

@@ -107,7 +107,8 @@ MIXED_DE_L = 1e-5  # Ha, |dE_L| target of the mixed path
 @pytest.mark.timeout(900)
 def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
     """The benchmark's configuration (split-BF16 contractions and propagator,
-    P W and P U with three-piece operands, FP64 LU/overlaps/energy sums,
+    P W and P U with three-piece operands and the two innermost Horner steps
+    with one-piece operands, FP64 LU/overlaps/energy sums,
     FP32-precision exchange in chunks of 2048, fields drawn on the device)
     and the same with two pieces throughout against the FP64 oracle, and the
     FP64 dense path against the oracle at 1e-10: one substep from the trial
@@ -124,7 +125,10 @@ def test_config_512_orbitals_mixed_vs_oracle(require_gpu):
         say(f"config 512 orbitals: GPU {key}, {nw} walkers, one substep from the trial")
         g = m.AfqmcSys(s, nw, mode, energy_precision=2, three_piece=three)
         if three:
+            # the benchmark's default: P W and P U in three pieces, U_6 and
+            # U_5 in one, the other Horner steps in two
             g.set_three_piece_steps(0x81)
+            g.set_one_piece_steps(0x06)
         g.set_exchange_chunk(2048)
         g.device_fields(seed)
         g.init_walkers()

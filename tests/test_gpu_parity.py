@@ -187,12 +187,12 @@ def test_energy_dense_exchange_equals_cholesky(c1, mode):
     close(out[0], out[1], 1e-12 if mode == "fp64_dense" else 1e-5, "E_L dense exchange vs Cholesky")
 
 
-def run_both(s, p, mode, nsteps=None, nw=None, cutoff=1e-6):
+def run_both(s, p, mode, nsteps=None, nw=None, cutoff=1e-6, csr_dense=True):
     nw = nw or p["nwalk"]
     ns = nsteps or p["nsteps"]
     nsub, northo = p["nsubsteps"], p["northo"]
     x = m.fields_block(p["seed"], s.nchol, nw, 0, ns * nsub)
-    g = m.AfqmcSys(s, nw, mode, cutoff=cutoff)
+    g = m.AfqmcSys(s, nw, mode, cutoff=cutoff, csr_dense=csr_dense)
     g.init_walkers()
     g.stage_fields(x)
     g.calculate_mixed_density_matrix()
@@ -207,10 +207,14 @@ def run_both(s, p, mode, nsteps=None, nw=None, cutoff=1e-6):
     return res, dict(eav=eo, W=Wo, Wd=Wdo, eshift=eso)
 
 
-def test_run_fp64_csr_matches_oracle(c1):
+@pytest.mark.parametrize("csr_dense", [True, False])
+def test_run_fp64_csr_matches_oracle(c1, csr_dense):
+    """The reference's algorithm over 20 steps; the half-rotated SpvnT of this
+    system is full, so the bias product runs on its dense copy unless
+    AFQMC_OPT_CSR_DENSE is 0 (then as the reference's CSR product)."""
     s, p = c1
-    say("run fp64_csr N16_M64 20 steps")
-    g, o = run_both(s, p, "fp64_csr")
+    say(f"run fp64_csr N16_M64 20 steps, csr_dense={csr_dense}")
+    g, o = run_both(s, p, "fp64_csr", csr_dense=csr_dense)
     close(g["eav"], o["eav"], 1e-9, "Eav trajectory")
     close(g["W"], o["W"], 1e-9, "final walkers")
     close(g["Wd"][:, 0], o["Wd"][:, 0], 1e-9, "final E_L")
@@ -703,3 +707,39 @@ def test_device_fields_equal_host_draws(c1, mode, graphs):
         g.close()
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+def test_csr_dense_bias_matches_csr_product(c1):
+    """AFQMC_OPT_CSR_DENSE: with SpvnT at least half full the FP64_CSR bias
+    product runs on a dense copy (entries under the cutoff zero), split into
+    chunks summed in FP64; it gives the CSR product's vbias to 1e-13, and a
+    sparser SpvnT (a cutoff that drops most entries) keeps the CSR product."""
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=13)
+    Wd = np.zeros((nw, 8), complex)
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    out = {}
+    for dense in (True, False):
+        g = m.AfqmcSys(s, nw, "fp64_csr", csr_dense=dense)
+        g.set(nat.BUF_G, Go)
+        out[dense] = g.get_vbias()
+        g.close()
+    close(out[True], out[False], 1e-13, "dense vs CSR bias product")
+    close(out[True], orc.get_vbias(orc.halfrotate(s), Go), RTOL, "dense bias product vs oracle")
+    spT = orc.halfrotate(s, cutoff=1e-6)
+    say(f"SpvnT density at cutoff 1e-6: {spT.nnz / (spT.nrows * spT.ncols):.3f}")
+    # which product ran: the call sites' kernel statistics
+    full = orc.halfrotate(s, cutoff=-1.0)
+    cut = float(np.quantile(np.abs(full.val), 0.7))  # keeps about 30 % of the entries
+    for cutoff, site in ((1e-6, "vbias_gemm"), (cut, "csrmm_vbias")):
+        g = m.AfqmcSys(s, nw, "fp64_csr", cutoff=cutoff)
+        g.set(nat.BUF_G, Go)
+        g.profile(True)
+        vb = g.get_vbias()
+        names = set(g.kernel_stats())
+        g.profile(False)
+        g.close()
+        say(f"cutoff {cutoff:.3g}: call sites {sorted(names)}")
+        assert site in names
+        close(vb, orc.get_vbias(orc.halfrotate(s, cutoff=cutoff), Go), RTOL, f"bias product at cutoff {cutoff:.3g}")
