@@ -1137,13 +1137,14 @@ int propagate_chain(afqmc_ctx* c, const void* P, const void* V)
   return AFQMC_OK;
 }
 
-// The propagator in three-piece split-BF16: W and S are transposed into
-// three-piece terms ([N][M] per walker and spin) so that every product reads
+// The propagator in three-piece split-BF16: W is transposed into three-piece
+// terms ([N][M] per walker and spin) and P W is written in that layout as
+// well (split, plus FP32 for the epilogues' F), so that every product reads
 // both operands k-contiguous through the staged-ring kernel.  Each product
-// runs in three pieces or, where `three_steps` says so, in two (the (h, m)
-// words of a three-piece operand are its two-piece split); a Horner term is
-// written with its third pieces only when the product that reads it takes
-// three.
+// runs in three, two or one pieces as `three_steps` / `one_steps` say (the
+// (h, m) words of a three-piece operand are its two-piece split, h alone its
+// one-piece form); a term is written with its third pieces only when the
+// product that reads it takes three.
 int propagate_three(afqmc_ctx* c)
 {
   const int M = c->M, N = c->N, nw = c->nw;
@@ -1171,29 +1172,28 @@ int propagate_three(afqmc_ctx* c)
     a.m = M;
     a.n = N;
     a.k = M;
+    a.scm = 1;  // terms transposed per walker and spin, [N][M]
+    a.scn = M;
+    a.sc1 = 2 * MN;
+    a.sc2 = MN;
     return a;
   };
-  {  // S = P W
+  {  // S = P W: U_7 = S split (third pieces when U_6's product takes three)
+     // into Ta, and S in FP32 in the same layout (D) for the Horner epilogues
     ZGemmArgs a = base();
     a.B = Tb;
     a.B3 = c->tp_Tb;
-    a.C = c->S;
-    a.scm = N;
-    a.scn = 1;
-    a.sc1 = 2 * MN;
-    a.sc2 = MN;
-    if (three(0)) RET((zgemm<CP_BF16X6, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
-    else RET((zgemm<CP_BF16X3, bfs, bfs, float, true, true>(c, "prop_pw", a, 2 * nw)));
+    a.C = Ta;
+    a.C3 = three(1) ? c->tp_Ta : nullptr;
+    a.D = c->S;
+    if (three(0)) RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_pw", a, 2 * nw)));
+    else RET((zgemm<CP_BF16X3, bfs, bfs, bfs, true, true>(c, "prop_pw", a, 2 * nw)));
   }
-  RET(launch(c, "prop_split_s", [&]() {
-    hipLaunchKernelGGL(transpose_split3_kernel<zf>, tgrid, dim3(256), 0, c->stream, (const zf*)c->S, Tb, c->tp_Tb, M,
-                       N);
-  }));
   // Horner's rule as propagate_chain: U_7 = S, U_k = S + (i/k) V U_{k+1}
-  zb* prev = Tb;
-  uint32_t* prev3 = c->tp_Tb;
-  zb* bufs[2] = {Ta, Tb};
-  uint32_t* bufs3[2] = {c->tp_Ta, c->tp_Tb};
+  zb* prev = Ta;
+  uint32_t* prev3 = c->tp_Ta;
+  zb* bufs[2] = {Tb, Ta};
+  uint32_t* bufs3[2] = {c->tp_Tb, c->tp_Ta};
   for (int k = 6; k >= 1; k--) {
     ZGemmArgs a = base();
     a.A = c->vHS;
@@ -1204,15 +1204,9 @@ int propagate_three(afqmc_ctx* c)
     a.B3 = prev3;
     a.C = bufs[k & 1];
     a.C3 = three(8 - k) ? bufs3[k & 1] : nullptr;
-    a.scm = 1;
-    a.scn = M;
-    a.sc1 = 2 * MN;
-    a.sc2 = MN;
     a.ar = 0.0;
     a.ai = 1.0 / (double)k;
-    a.F = c->S;
-    a.sfm = N;
-    a.sfn = 1;
+    a.F = c->S;  // same layout as C
     a.cf = 1;
     if (three(7 - k)) RET((zgemm<CP_BF16X6, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
     else if (one(7 - k)) RET((zgemm<CP_BF16X1, bfs, bfs, bfs, true, true>(c, "prop_taylor", a, 2 * nw)));
@@ -1227,8 +1221,6 @@ int propagate_three(afqmc_ctx* c)
     a.C = c->W;
     a.scm = N;
     a.scn = 1;
-    a.sc1 = 2 * MN;
-    a.sc2 = MN;
     if (three(7)) RET((zgemm<CP_BF16X6, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
     else RET((zgemm<CP_BF16X3, bfs, bfs, double, true, true>(c, "prop_ps", a, 2 * nw)));
   }

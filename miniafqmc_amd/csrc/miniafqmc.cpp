@@ -38,7 +38,7 @@ const TimerSites kTimers[] = {
     {"Bias Potential", {"vbias_gemm", "vbias_reduce", "csrmm_vbias", "csrmm_vbias_t"}},
     {"H-S Potential", {"vhs_gemm", "csrmm_vhs", "vhs_transpose"}},
     {"Sigma", {"fields", "set_substep"}},
-    {"Propagation", {"prop_pw", "prop_taylor", "prop_ps", "prop_split_w", "prop_split_s", "prop_fused"}},
+    {"Propagation", {"prop_pw", "prop_taylor", "prop_ps", "prop_split_w", "prop_fused"}},
     {"Other", {"save_overlaps", "weights", "branch_select", "branch_gather"}},
     {"Overlap", {"ovlp_gemm", "gj_inverse"}},
     {"Orthgonalization",
