@@ -172,7 +172,11 @@ __global__ __launch_bounds__(NMAX* CG) void gj_inverse_kernel(const zd* __restri
           for (int t = 0; t < NC; t++) x[t] = oldk[cg + CG * t];
           xc = oldk[jc];
         }
-        const zd f = swapped_in ? colk[k] : colk[i];
+        // X(i,k) of the row now at i: the swapped-in row takes old row k's,
+        // from oldk (written between this step's barriers), not colk[k],
+        // which row k's owner overwrites at the start of the next step with
+        // no barrier in between (row k may sit in another wave)
+        const zd f = swapped_in ? oldk[k] : colk[i];
         const zd gq = zmul(zd{-f.re, -f.im}, ipv);  // -X(i,k)/piv
 #pragma unroll
         for (int t = 0; t < NC; t++) {
