@@ -65,15 +65,32 @@ def csr_flops(name, nw, nnz_spvn, nnz_spvnT):
     return None if c is None else 8.0 * c
 
 
-def kernel_flops(name, M, N, K, nw):
-    """Algorithmic real flops per launch (8 per complex multiply-add)."""
+def contraction_spins_per_launch(M, N, K, nw):
+    """Spin blocks per launch of the bias-shaped contraction (G against the
+    K Cholesky rows, the bias and the Coulomb term).  The library splits its
+    k = N*M range into S chunks, S doubling from 1 while 2*S*tiles < 2048
+    (tiles = ceil(nw/128) * ceil(K/64)) and N*M/(2S) stays a multiple of 64;
+    with S > 1 one launch covers both spins, otherwise each spin is its own
+    launch."""
+    NM = N * M
+    tiles = -(-nw // 128) * -(-K // 64)
+    S = 1
+    while S < 16 and 2 * S * tiles < 2048 and NM % (2 * S) == 0 and (NM // (2 * S)) % 64 == 0:
+        S *= 2
+    return 2 if S > 1 else 1
+
+
+def kernel_flops(name, M, N, K, nw, mode=None):
+    """Algorithmic real flops per launch (8 per complex multiply-add).  The
+    FP64 CSR mode's dense bias product contracts both spins in one launch."""
+    sp = 2 if mode == "fp64_csr" else contraction_spins_per_launch(M, N, K, nw)
     c = {
         "vhs_gemm": nw * M * M * K,
         "prop_taylor": 2 * nw * M * M * N,
         "prop_pw": 2 * nw * M * M * N,
         "prop_ps": 2 * nw * M * M * N,
-        "vbias_gemm": nw * K * N * M,       # one spin block per launch
-        "energy_v_gemm": nw * K * N * M,
+        "vbias_gemm": sp * nw * K * N * M,
+        "energy_v_gemm": sp * nw * K * N * M,
         "ovlp_gemm": 2 * nw * N * N * M,
         "gc_gemm": 2 * nw * N * N * M,
         "gj_inverse": 2 * nw * N * N * N,
@@ -81,7 +98,7 @@ def kernel_flops(name, M, N, K, nw):
     return None if c is None else 8.0 * c
 
 
-def kernel_flops_total(name, launches, M, N, K, nw, n_energy):
+def kernel_flops_total(name, launches, M, N, K, nw, n_energy, mode=None):
     """Algorithmic real flops of all `launches` of a call site.  The exchange
     GEMM's launches differ in size (column chunks of the lower triangle), so
     its total is counted per energy evaluation: both spins, every walker,
@@ -90,7 +107,7 @@ def kernel_flops_total(name, launches, M, N, K, nw, n_energy):
     if name == "energy_vx_gemm":
         NM = N * M
         return 8.0 * n_energy * 2 * nw * NM * (NM + 1) / 2
-    f = kernel_flops(name, M, N, K, nw)
+    f = kernel_flops(name, M, N, K, nw, mode)
     return None if f is None else f * launches
 
 
@@ -408,7 +425,7 @@ def main():
     else:
         rows = []
         for k_, (nl, ms) in stats.items():
-            fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy)
+            fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy, args.mode)
             if fl is None or nl == 0 or ms <= 0:
                 continue
             unit, pk = kernel_peak(k_, args.mode, args.energy_level)

@@ -200,12 +200,17 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
       const uint4 v0 = t4[(arow + 16 * i) * 8 + c0], v1 = t4[(arow + 16 * i) * 8 + c1];
       f.ah[i] = make_uint4(v0.x, v0.z, v1.x, v1.z);
       f.al[i] = make_uint4(v0.y, v0.w, v1.y, v1.w);
+      // one piece uses only the hi words: keep the lo words live so that the
+      // reads stay 16 bytes wide (narrowed reads of every other word conflict)
+      if constexpr (NP == 1) asm volatile("" ::"v"(v0.y), "v"(v0.w), "v"(v1.y), "v"(v1.w));
     }
     if constexpr (NP != 3) {
 #pragma unroll
       for (int j = 0; j < TN; j++) {
         f.b0[j] = t4[(brow + 16 * j) * 8 + c0];
         f.b1[j] = t4[(brow + 16 * j) * 8 + c1];
+        if constexpr (NP == 1)
+          asm volatile("" ::"v"(f.b0[j].y), "v"(f.b0[j].w), "v"(f.b1[j].y), "v"(f.b1[j].w));
       }
     } else {
       const uint4* t3 = (const uint4*)(st + (BM + BN) * KT);  // 4 uint4 per third-piece row

@@ -3,6 +3,7 @@
 FP64 bar: 1e-10 relative (per element, relative to the array's max
 magnitude) for every phase; the multi-step trajectory compares at 1e-9.
 """
+import os
 import sys
 
 import numpy as np
@@ -466,6 +467,30 @@ def test_branch_between_graph_replays(c1):
         close(ctxs[1].W_data[:, :5], ctxs[0].W_data[:, :5], 1e-13, f"W_data after substep {t}")
     for g in ctxs:
         g.close()
+
+
+@pytest.mark.parametrize("M,N,K,nw", [(64, 16, 256, 16), (40, 7, 53, 5)])
+@pytest.mark.parametrize("mode", ["fp64_csr", "fp64_dense", "mixed_bf16"])
+def test_bias_contraction_spins_per_launch(require_gpu, M, N, K, nw, mode):
+    """The bench's algorithmic flop count of the bias product assumes one or
+    two spin blocks per launch as the library's split-K choice makes them:
+    one bias evaluation takes 2 / spins launches of vbias_gemm."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    say("bias launches", M, N, K, nw, mode)
+    s = m.make_system(M, N, K, density=0.5, sigma_L=0.03, sigma_h=1.0, dt=0.005, seed=7, with_vakbl=False)
+    g = m.AfqmcSys(s, nw, mode, cutoff=1e-6 if mode == "fp64_csr" else -1.0)
+    g.profile(True)
+    g.get_vbias()
+    st = g.kernel_stats()
+    g.profile(False)
+    g.close()
+    spins = 2 if mode == "fp64_csr" else bench.contraction_spins_per_launch(M, N, K, nw)
+    f1 = bench.kernel_flops("vbias_gemm", M, N, K, nw, mode)
+    say(f"vbias_gemm launches {st.get('vbias_gemm')}, spins per launch {spins}")
+    assert st["vbias_gemm"][0] == 2 // spins
+    # the flops of one evaluation are both spins' whatever the split
+    assert f1 * st["vbias_gemm"][0] == 8.0 * 2 * nw * K * N * M
 
 
 @pytest.mark.parametrize("M,N,K,nw", [(40, 7, 53, 5), (24, 1, 10, 3), (33, 12, 70, 1)])
