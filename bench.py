@@ -32,6 +32,7 @@ PEAKS = {  # MI355X published dense peaks, TFLOP/s
     "fp32": 157.3,
     "bf16": 2500.0,
 }
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
 # the headline metric: the mixed-precision step at 512 orbitals, 128+128
 # electrons and 2048 Cholesky vectors, compared with the FP64 path at a
 # per-walker local-energy tolerance of 1e-5 Ha
@@ -44,9 +45,26 @@ MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128"
 # counters, scaled by a 1 GiB copy), per (call site, workload, mode);
 # profiles/walker_step_N128_M512_hbm_traffic.md has the commands and figures.
 MEASURED_TRAFFIC = {
-    # Horner step: 4.03 GiB read (V, U, S) + 1.00 GiB written (U') per launch
-    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.403e9,
+    # Horner steps, average of the six: 4.02 GiB read (V, U, S) + 1.09 GiB
+    # written (U', and U1's third pieces) per launch
+    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.489e9,
 }
+
+
+def mixed_alg_bytes(name, M, N, K, nw, third_u1):
+    """Algorithmic HBM bytes per launch of the split-BF16 call sites whose
+    operands stream through once (8 B per complex in the split form, the
+    same as a complex FP32): the Horner step reads V (M x M), U (M x 2N) and
+    S as its FP32 addend, and writes U' (M x 2N); when P U takes three pieces
+    the last of the six also writes U's third pieces (4 B per complex), so
+    the average launch carries a sixth of that.  The HS potential reads the
+    K Cholesky matrices and the fields and writes vHS."""
+    if name == "prop_taylor":
+        u = 8 * nw * M * 2 * N
+        return 8 * nw * M * M + 3 * u + (u / 2 / 6 if third_u1 else 0)
+    if name == "vhs_gemm":
+        return 8 * K * M * M + 8 * K * nw + 8 * nw * M * M
+    return None
 
 
 def kernel_bytes(name, M, N, K, nw, nnz_spvn, nnz_spvnT):
@@ -109,6 +127,50 @@ def kernel_flops_total(name, launches, M, N, K, nw, n_energy, mode=None):
         return 8.0 * n_energy * 2 * nw * NM * (NM + 1) / 2
     f = kernel_flops(name, M, N, K, nw, mode)
     return None if f is None else f * launches
+
+
+def roofline_rows(stats, args, M, N, K, nw, n_energy):
+    """Roofline rows of the call sites with a known algorithmic count, most
+    total time first, and the bench line's `roofline` entry for the first.
+    A call site's bound is the roof its algorithmic work reaches last: HBM
+    when its bytes at 8 TB/s take longer than its flops at the unit's peak
+    (an intensity below the ridge point), the MFMA unit otherwise; both
+    fractions are kept."""
+    third_u1 = args.mode == "mixed_bf16" and bool(args.three_piece & 4) and bool(args.three_steps & 0x80)
+    rows = []
+    for k_, (nl, ms) in stats.items():
+        fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy, args.mode)
+        if fl is None or nl == 0 or ms <= 0:
+            continue
+        unit, pk = kernel_peak(k_, args.mode, args.energy_level)
+        avg_s = ms / nl / 1e3
+        ach = fl / (ms / 1e3) / 1e12
+        r = {"kernel": k_, "achieved": round(ach, 2), "peak": pk, "peak_unit": unit,
+             "frac": round(ach / pk, 4), "ms_total": round(ms, 2), "launches": nl,
+             "avg_ms": round(ms / nl, 4), "flops_per_launch": fl / nl, "bound": "mfma", "unit": "TFLOP/s",
+             "mfma_frac": round(ach / pk, 4), "tflops": ach, "mfma_unit": unit}
+        nb = mixed_alg_bytes(k_, M, N, K, nw, third_u1) if args.mode == "mixed_bf16" else None
+        if nb is not None:
+            gbs = nb / avg_s / 1e9
+            r["bytes_per_launch"] = nb
+            r["hbm_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+            if nb / (HBM_PEAK_GBS * 1e9) > (fl / nl) / (pk * 1e12):
+                r.update(bound="hbm", unit="GB/s", achieved=round(gbs, 1), peak=HBM_PEAK_GBS,
+                         peak_unit="hbm", frac=r["hbm_frac"])
+        rows.append(r)
+    rows.sort(key=lambda r: -r["ms_total"])
+    if not rows:
+        return None, rows
+    top = rows[0]
+    roof = {"kernel": top["kernel"], "bound": top["bound"], "achieved": top["achieved"], "peak": top["peak"],
+            "unit": top["unit"], "frac": top["frac"],
+            "traffic": MEASURED_TRAFFIC.get((top["kernel"], args.workload, args.mode)),
+            "avg_ms": top["avg_ms"], "flops_per_launch": top["flops_per_launch"],
+            "bytes_per_launch": top.get("bytes_per_launch"), "mfma_frac": top["mfma_frac"],
+            "hbm_frac": top.get("hbm_frac"),
+            "kernels": [{k_: r[k_] for k_ in ("kernel", "bound", "achieved", "peak", "peak_unit", "frac",
+                                              "mfma_frac", "ms_total", "launches")} for r in rows[:3]]}
+    return roof, rows
 
 
 def kernel_peak(name, mode, energy_level):
@@ -418,33 +480,16 @@ def main():
         nbytes = kernel_bytes(name, M, N, K, nw, s.spvn.nnz, res["nnz_spvnT"])
         fl = csr_flops(name, nw, s.spvn.nnz, res["nnz_spvnT"])
         ach = nbytes / avg_s / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(ach / 8000.0, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
                 "avg_ms": round(ms / nl, 4), "bytes_per_launch": nbytes,
                 "fp64_tflops": round(fl / avg_s / 1e12, 2)}
     else:
-        rows = []
-        for k_, (nl, ms) in stats.items():
-            fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy, args.mode)
-            if fl is None or nl == 0 or ms <= 0:
-                continue
-            unit, pk = kernel_peak(k_, args.mode, args.energy_level)
-            ach = fl / (ms / 1e3) / 1e12
-            rows.append({"kernel": k_, "achieved": round(ach, 2), "peak": pk, "peak_unit": unit,
-                         "frac": round(ach / pk, 4), "ms_total": round(ms, 2), "launches": nl,
-                         "avg_ms": round(ms / nl, 4), "flops_per_launch": fl / nl})
-        rows.sort(key=lambda r: -r["ms_total"])
-        if rows:
-            top = rows[0]
-            roof = {"kernel": top["kernel"], "bound": "mfma", "achieved": top["achieved"], "peak": top["peak"],
-                    "unit": "TFLOP/s", "frac": top["frac"],
-                    "traffic": MEASURED_TRAFFIC.get((top["kernel"], args.workload, args.mode)),
-                    "avg_ms": top["avg_ms"], "flops_per_launch": top["flops_per_launch"],
-                    "kernels": [{k_: r[k_] for k_ in ("kernel", "achieved", "peak", "peak_unit", "frac", "ms_total",
-                                                      "launches")} for r in rows[:3]]}
-            for r in rows:
-                print(f"[bench] roofline {r['kernel']:16s} {r['achieved']:8.1f} TFLOP/s of {r['peak']:.1f} "
-                      f"({r['peak_unit']}) = {r['frac']:.3f}, {r['ms_total']:.1f} ms", flush=True)
+        roof, rows = roofline_rows(stats, args, M, N, K, nw, n_energy)
+        for r in rows:
+            hb = f", {r['hbm_frac']:.3f} of HBM" if "hbm_frac" in r else ""
+            print(f"[bench] roofline {r['kernel']:16s} bound {r['bound']:4s} {r['tflops']:8.1f} TFLOP/s = "
+                  f"{r['mfma_frac']:.3f} of {r['mfma_unit']}{hb}; {r['ms_total']:.1f} ms", flush=True)
 
     extra = {}
     if stats:

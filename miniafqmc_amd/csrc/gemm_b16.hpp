@@ -26,9 +26,13 @@
 // expanded form, so both kernels give identical results.
 //
 // NP = 3 (three-piece split-BF16, 24 significant bits): each operand's third
-// pieces (one word per complex, p.A3 / p.B3) get LDS tiles of their own (rows
-// of 64 bytes, chunk c of row R at c ^ ((R >> 2) & 3), conflict-free for the
-// 16 rows of a fragment read), and the six products h*h, h*m, m*h, h*l, l*h,
+// pieces (one word per complex, p.A3 / p.B3; 64 bytes per row and k-tile)
+// get LDS tiles of their own, two 16-row blocks to a 128-byte LDS row: row
+// r of the blocks at rows 32q and 32q + 16 goes to LDS row 16q + r, its
+// chunk c at (2c) ^ swz(r) and (2c + 1) ^ swz(r) respectively.  A fragment
+// read of one block is then the two-piece tiles' read of chunk pair c0 / c1
+// (16 rows of 64-byte rows read densely conflict two ways on this LDS, the
+// swizzled half-rows do not).  The six products h*h, h*m, m*h, h*l, l*h,
 // m*m accumulate in that order.
 //
 // NP = 1 (one piece, 8 significant bits): the same two-piece tiles are staged
@@ -47,7 +51,6 @@ template <int N> __device__ __forceinline__ void wait_vmcnt()
 }
 
 __device__ __forceinline__ int swz(int row) { return (row & 7) ^ ((row >> 3) & 1); }
-__device__ __forceinline__ int swz3(int row) { return (row >> 2) & 3; }
 
 } // namespace b16
 
@@ -74,6 +77,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
   constexpr int STG = (BM + BN) * KT + (NP == 3 ? (BM + BN) * KT / 2 : 0);
   static_assert(NP >= 1 && NP <= 3, "one, two or three pieces");
   static_assert(NP != 3 || ((BM * C3) % NT == 0 && (BN * C3) % NT == 0), "third-piece tiles must split evenly");
+  static_assert(NP != 3 || (WTM % 32 == 0 && WTN % 32 == 0), "third pieces pair 16-row blocks within a wave tile");
   static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile must be a multiple of 16");
   static_assert((BM * CH) % NT == 0 && (BN * CH) % NT == 0, "tiles must split evenly over the threads");
   static_assert(STAGES >= 2 && STAGES <= 4, "2 to 4 stages");
@@ -129,18 +133,22 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
     const int u = i * NT + tid, row = u / CH, pc = u % CH;
     pb[i] = B + (long)min(n0 + row, p.n - 1) * p.sbn + 2 * (pc ^ b16::swz(row));
   }
-  // third pieces: unit u = (row u / C3, chunk u % C3), four complex per chunk
+  // third pieces: unit u is slot u % 8 of LDS row u / 8 = 16q + r, which
+  // holds chunk x / 2 (four complex) of tile row 32q + 16 (x & 1) + r, where
+  // x = (u % 8) ^ swz(r)
+  auto row3 = [](int u) { const int sr = u >> 3, x = (u & 7) ^ b16::swz(sr & 15); return 32 * (sr >> 4) + 16 * (x & 1) + (sr & 15); };
+  auto chunk3 = [](int u) { return (((u & 7) ^ b16::swz((u >> 3) & 15)) >> 1); };
   const uint32_t* pa3[AL3 > 0 ? AL3 : 1];
   const uint32_t* pb3[BL3 > 0 ? BL3 : 1];
 #pragma unroll
   for (int i = 0; i < AL3; i++) {
-    const int u = i * NT + tid, row = u / C3, pc = u % C3;
-    pa3[i] = A3 + (long)min(m0 + row, p.m - 1) * p.sam + 4 * (pc ^ b16::swz3(row));
+    const int u = i * NT + tid;
+    pa3[i] = A3 + (long)min(m0 + row3(u), p.m - 1) * p.sam + 4 * chunk3(u);
   }
 #pragma unroll
   for (int i = 0; i < BL3; i++) {
-    const int u = i * NT + tid, row = u / C3, pc = u % C3;
-    pb3[i] = B3 + (long)min(n0 + row, p.n - 1) * p.sbn + 4 * (pc ^ b16::swz3(row));
+    const int u = i * NT + tid;
+    pb3[i] = B3 + (long)min(n0 + row3(u), p.n - 1) * p.sbn + 4 * chunk3(u);
   }
   const int w0 = __builtin_amdgcn_readfirstlane(wave);
   auto issue = [&](uint2* st, int k0) __attribute__((always_inline)) {
@@ -182,7 +190,9 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
   const int sw = b16::swz(r);
   const int c0 = (2 * g) ^ sw, c1 = (2 * g + 1) ^ sw;
   const int arow = wm * WTM + r, brow = BM + wn * WTN + r;
-  const int c3 = g ^ b16::swz3(r);  // third-piece chunk of this lane's four k
+  // third pieces: LDS rows of the wave tile's first block pair (A, then B
+  // after A's BM / 2 rows); block i of the pair reads chunk pair c0 / c1
+  const int arow3 = wm * WTM / 2 + r, brow3 = BM / 2 + wn * WTN / 2 + r;
   // fragments of one k-tile: A rows (hi and lo words of four complex k) and
   // the raw B columns, read from LDS before the next loads are issued
   // (three pieces: the B columns are read per column block inside the MFMA
@@ -213,9 +223,9 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
           asm volatile("" ::"v"(f.b0[j].y), "v"(f.b0[j].w), "v"(f.b1[j].y), "v"(f.b1[j].w));
       }
     } else {
-      const uint4* t3 = (const uint4*)(st + (BM + BN) * KT);  // 4 uint4 per third-piece row
+      const uint4* t3 = (const uint4*)(st + (BM + BN) * KT);  // 8 uint4 per LDS row
 #pragma unroll
-      for (int i = 0; i < TM; i++) f.a3[i] = t3[(arow + 16 * i) * 4 + c3];
+      for (int i = 0; i < TM; i++) f.a3[i] = t3[(arow3 + 16 * (i >> 1)) * 8 + ((i & 1) ? c1 : c0)];
     }
   };
   auto mfmas = [&](const Frags& f, const uint2* st) __attribute__((always_inline)) {
@@ -267,7 +277,7 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
             for (int i = 0; i < TM; i++) mm(i, j, t == 0 ? f.ah[i] : f.al[i], rl, il);
         }
         {
-          const uint4 w = ((const uint4*)(st + (BM + BN) * KT))[(brow + 16 * j) * 4 + c3];
+          const uint4 w = ((const uint4*)(st + (BM + BN) * KT))[(brow3 + 16 * (j >> 1)) * 8 + ((j & 1) ? c1 : c0)];
           const uint4 r3 = make_uint4(ng(w.x), ng(w.y), ng(w.z), ng(w.w));
           const uint4 i3 = make_uint4(rt(w.x), rt(w.y), rt(w.z), rt(w.w));
 #pragma unroll

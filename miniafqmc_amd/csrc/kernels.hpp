@@ -1378,6 +1378,9 @@ __global__ __launch_bounds__(NMAX* CG) void chol_inv_kernel(const zd* __restrict
     if (tid == 0) fail[b] = 1;
     return;
   }
+  // the last Cholesky step read prow[(N - 1) & 1], which the inverse's first
+  // step (buffer 0) overwrites before its own barrier when N is odd
+  __syncthreads();
   // rows i < N now hold R (upper); rows >= N the identity.  In-place
   // Gauss-Jordan inverse, pivots on the diagonal, the columns in blocks of CG
   // as gj_inverse_kernel: slot tk of block tk is held in xc, addressed

@@ -49,3 +49,38 @@ def test_reference_timers_cover_every_call_site():
         "Propagation", "Other", "Overlap", "Orthgonalization", "Local Energy"]
     got = m.reference_timers({"prop_taylor": (6, 12.0), "prop_pw": (1, 2.0), "gj_inverse": (1, 5.0), "x": (1, 1.0)})
     assert got["Propagation"] == (7, 14.0) and got["Overlap"] == (1, 5.0) and got["Sigma"] == (0, 0)
+
+
+def test_roofline_bound_follows_intensity():
+    """The roofline entry takes the roof a call site's algorithmic work
+    reaches last.  A Horner step at 512 orbitals / 128+128 electrons / 1024
+    walkers streams V (2 MiB per walker in the split form) and U, S and U'
+    (1 MiB each), plus U1's third pieces (0.5 MiB) once in six launches,
+    for 8 * 2 * 512^2 * 128 flops per walker: about 101 flop/B, below the
+    BF16 ridge (312), so HBM bounds it; the HS potential (681 flop/B) and
+    the exchange stay MFMA-bound."""
+    import argparse
+    sys.path.insert(0, ROOT)
+    import bench
+    M, N, K, nw = 512, 128, 2048, 1024
+    args = argparse.Namespace(mode="mixed_bf16", three_piece=4, three_steps=0x81, energy_level=2,
+                              workload="N128_M512")
+    stats = {"prop_taylor": (120, 120 * 1.6726), "vhs_gemm": (20, 20 * 9.56), "energy_vx_gemm": (64, 149.7),
+             "fields": (20, 0.3)}
+    roof, rows = bench.roofline_rows(stats, args, M, N, K, nw, n_energy=1)
+    assert [r["kernel"] for r in rows] == ["prop_taylor", "vhs_gemm", "energy_vx_gemm"]
+    mib = 1 << 20
+    nbytes = nw * (2 + 3 + 0.5 / 6) * mib
+    assert roof["kernel"] == "prop_taylor" and roof["bound"] == "hbm" and roof["unit"] == "GB/s"
+    assert abs(roof["bytes_per_launch"] - nbytes) < 1.0
+    assert abs(roof["achieved"] - nbytes / 1.6726e-3 / 1e9) < 0.1
+    assert abs(roof["frac"] - nbytes / 1.6726e-3 / 8e12) < 1e-4
+    flops = 8.0 * 2 * nw * M * M * N
+    assert abs(roof["mfma_frac"] - flops / 1.6726e-3 / 2.5e15) < 1e-4
+    by = {r["kernel"]: r for r in rows}
+    assert by["vhs_gemm"]["bound"] == "mfma" and by["energy_vx_gemm"]["bound"] == "mfma"
+    assert "hbm_frac" not in by["energy_vx_gemm"]
+    # the FP32 mode reports the MFMA view only
+    args.mode = "mixed_fp32"
+    roof32, _ = bench.roofline_rows(stats, args, M, N, K, nw, n_energy=1)
+    assert roof32["bound"] == "mfma"
