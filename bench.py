@@ -142,7 +142,7 @@ def roofline_rows(stats, args, M, N, K, nw, n_energy):
         fl = kernel_flops_total(k_, nl, M, N, K, nw, n_energy, args.mode)
         if fl is None or nl == 0 or ms <= 0:
             continue
-        unit, pk = kernel_peak(k_, args.mode, args.energy_level)
+        unit, pk = kernel_peak(k_, args.mode, args.energy_level, bool(getattr(args, "coulomb3", 0)))
         avg_s = ms / nl / 1e3
         ach = fl / (ms / 1e3) / 1e12
         r = {"kernel": k_, "achieved": round(ach, 2), "peak": pk, "peak_unit": unit,
@@ -173,8 +173,9 @@ def roofline_rows(stats, args, M, N, K, nw, n_energy):
     return roof, rows
 
 
-def kernel_peak(name, mode, energy_level):
-    """Peak (TFLOP/s) of the unit a call site runs on in a precision mode."""
+def kernel_peak(name, mode, energy_level, coulomb3=False):
+    """Peak (TFLOP/s) of the unit a call site runs on in a precision mode
+    (coulomb3: the Coulomb contraction in three-piece split-BF16)."""
     if mode in ("fp64_csr", "fp64_dense") or name in ("ovlp_gemm", "gj_inverse"):
         return "fp64", PEAKS["fp64"]
     if name == "energy_vx_gemm":
@@ -182,7 +183,7 @@ def kernel_peak(name, mode, energy_level):
         u = {0: MODE_PEAK[mode], 1: MODE_PEAK[mode], 2: "bf16" if mode == "mixed_bf16" else "fp32"}.get(
             energy_level, "fp64")
         return u, PEAKS[u]
-    if name == "energy_v_gemm" and energy_level >= 1:
+    if name == "energy_v_gemm" and energy_level >= 1 and not (coulomb3 and mode == "mixed_bf16" and energy_level == 2):
         return "fp64", PEAKS["fp64"]
     return MODE_PEAK[mode], PEAKS[MODE_PEAK[mode]]
 
@@ -318,6 +319,9 @@ def main():
     ap.add_argument("--one-steps", type=lambda v: int(v, 0), default=0x06,
                     help="with the propagator in three pieces: a mask of the Horner steps (bits 1-6: U_6 .. U_1) "
                          "whose products take one piece (the hi bf16 words)")
+    ap.add_argument("--coulomb3", type=int, default=0,
+                    help="split-BF16 mode at energy level 2: the local energy's Coulomb contraction in three-piece "
+                         "split-BF16 instead of FP64")
     ap.add_argument("--variant", type=int, default=1,
                     help="split-BF16 mode: also time the other piece count (two pieces if --three-piece is set, "
                          "else all three-piece) from the same walkers and report it beside the value")
@@ -400,7 +404,8 @@ def main():
         from `state`, or from this run's own final walkers (returned)."""
         three = args.three_piece if three is None else three
         g = m.AfqmcSys(s, nw, mode, energy_precision=args.energy_level,
-                       three_piece=three if mode == "mixed_bf16" else 0)
+                       three_piece=three if mode == "mixed_bf16" else 0,
+                       coulomb_three_piece=bool(args.coulomb3) and mode == "mixed_bf16")
         g.set_exchange_chunk(args.exchange_chunk)
         g.set_direct_lds(args.direct_lds)
         if mode == "mixed_bf16" and three & 4:
@@ -608,6 +613,8 @@ def main():
                    else None,
                    "one_piece_steps": args.one_steps if args.mode == "mixed_bf16" and args.three_piece & 4
                    else None,
+                   "coulomb": ("three-piece split-BF16" if args.coulomb3 and args.mode == "mixed_bf16"
+                               and args.energy_level == 2 else "FP64" if args.energy_level >= 1 else "contraction"),
                    "energy": "none" if args.no_energy else f"once per {spe} substeps",
                    "parallelism": "single GPU"},
         "roofline": roof,

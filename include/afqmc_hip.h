@@ -163,7 +163,15 @@ typedef enum {
    * half of its entries are stored (those under the cutoff are zero) and
    * runs the bias product on FP64 MFMA; 0 always runs the CSR product.  No
    * counterpart in the reference (its csrmm is the only form). */
-  AFQMC_OPT_CSR_DENSE = 11
+  AFQMC_OPT_CSR_DENSE = 11,
+  /* nonzero (AFQMC_MIXED_BF16 at energy levels 1-2, set before the system
+   * upload): the local energy's Coulomb term contracts the Cholesky vectors
+   * with G in three-piece split-BF16 (24-bit operands, FP32 accumulation
+   * over chunks summed in FP64) instead of FP64; G itself stays FP64 for the
+   * one-body term.  0 (the default) keeps the FP64 contraction.  No
+   * counterpart in the reference (its energy is FP64 throughout,
+   * src/AFQMC/energy.hpp:63-102). */
+  AFQMC_OPT_COULOMB_THREE_PIECE = 12
 } afqmc_option;
 int afqmc_set_option(afqmc_ctx* ctx, int option, double value);
 

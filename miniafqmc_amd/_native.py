@@ -30,7 +30,7 @@ MODES = {"fp64_csr": 0, "fp64_dense": 1, "mixed_fp32": 2, "mixed_bf16": 3}
 BUF_W, BUF_WDATA, BUF_G, BUF_VBIAS, BUF_X, BUF_VHS, BUF_HYBRIDW, BUF_EPARTS, BUF_GFULL = range(9)
 (OPT_DENSE_EXCHANGE, OPT_ENERGY_PRECISION, OPT_ORTHO_HOUSEHOLDER, OPT_TRANSPOSED, OPT_EXCHANGE_CHUNK,
  OPT_FUSED_PROPAGATOR, OPT_DIRECT_LDS, OPT_TILE, OPT_THREE_PIECE, OPT_THREE_PIECE_STEPS,
- OPT_ONE_PIECE_STEPS, OPT_CSR_DENSE) = range(12)
+ OPT_ONE_PIECE_STEPS, OPT_CSR_DENSE, OPT_COULOMB_THREE_PIECE) = range(13)
 
 # every symbol include/afqmc_hip.h declares
 EXPORTED = [

@@ -37,7 +37,7 @@ REFERENCE_TIMERS = (
     ("Orthgonalization", ("ortho_gram", "ortho_chol_inv", "ortho_sign", "ortho_apply", "ortho_flags",
                           "lq_orthogonalize", "lq_fallback")),
     ("Local Energy", ("energy_one", "energy_two", "energy_direct", "energy_exch_dot", "energy_pairsum",
-                      "energy_assemble", "energy_zero", "energy_trace", "energy_v_gemm", "energy_vx_gemm",
+                      "energy_assemble", "energy_zero", "energy_trace", "energy_v_gemm", "energy_v_reduce", "energy_vx_gemm",
                       "energy_y_gemm", "csrmm_vakbl", "g_transpose", "gc64_gemm", "convert", "g_split3")),
 )
 
@@ -57,7 +57,8 @@ def reference_timers(stats):
 class AfqmcSys:
     def __init__(self, system: System, nwalk: int, mode: str = "fp64_csr", cutoff: float = 1e-6,
                  use_vakbl: bool = True, dense_exchange: bool = True, energy_precision: int = 1,
-                 transposed: bool = True, three_piece: int = 0, csr_dense: bool = True):
+                 transposed: bool = True, three_piece: int = 0, csr_dense: bool = True,
+                 coulomb_three_piece: bool = False):
         self.sys = system
         self.M, self.N, self.K, self.nwalk = system.nmo, system.naea, system.nchol, nwalk
         self.mode = mode
@@ -73,6 +74,9 @@ class AfqmcSys:
         if three_piece:
             # split-BF16 mode: contractions in three-piece split-BF16 (1 bias, 2 vHS, 4 propagator)
             self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_THREE_PIECE, float(three_piece)))
+        if coulomb_three_piece:
+            # split-BF16 mode, energy level 2: the Coulomb term's contraction in three pieces
+            self._check(nat.lib.afqmc_set_option(self.ctx, nat.OPT_COULOMB_THREE_PIECE, 1.0))
         ta = np.ascontiguousarray(system.trial_alpha)
         tb = np.ascontiguousarray(system.trial_beta)
         sp = system.spvn.view()

@@ -72,6 +72,7 @@ struct afqmc_ctx {
   // of it is stored (AFQMC_OPT_CSR_DENSE), for the bias product on MFMA
   zd* sptd = nullptr;
   bool csr_dense = true;
+  bool coulomb3 = false;  // AFQMC_OPT_COULOMB_THREE_PIECE
   // non-transposed bias (miniafqmc_base.cpp:267-276): T(Spvn) as CSR [K][M*M]
   // and the full density matrix G [2*M*M][nw]
   bool transposed = true;
@@ -802,7 +803,8 @@ int do_overlaps(afqmc_ctx* c)
 // workspace and are then summed in FP64.
 template <int CP, class GT, class LT_>
 int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, const char* name,
-                         const uint32_t* G3 = nullptr, const uint32_t* Lt3 = nullptr)
+                         const uint32_t* G3 = nullptr, const uint32_t* Lt3 = nullptr,
+                         const char* rname = "vbias_reduce")
 {
   const long NM = (long)c->N * c->M, nw = c->nw, K = c->K;
   const long tiles = ((nw + 127) / 128) * ((K + 63) / 64);
@@ -865,7 +867,7 @@ int dense_contract_vbias(afqmc_ctx* c, const void* Gp, const void* Lt, zd* out, 
   a.A3 = G3;  // third pieces with the same offsets
   a.B3 = Lt3;
   RET((zgemm<CP, GT, LT_, PT, true, true>(c, name, a, 2 * S)));
-  return launch(c, "vbias_reduce", [&]() {
+  return launch(c, rname, [&]() {
     hipLaunchKernelGGL((zsum_slices_kernel<PZ>), dim3(nblk(nw * K, 256)), dim3(256), 0, c->stream, (const PZ*)c->vws, out,
                        nw * K, 2 * S);
   });
@@ -1458,7 +1460,14 @@ int energy_cholesky(afqmc_ctx* c, const void* Gp, const void* Lv, const void* Lx
     (void)hipMemsetAsync(c->e_exch_part, 0, sizeof(zd) * 2 * nw, c->stream);
   }));
   // direct term: v(w,n) = sum Lt G ; direct = sum_n v^2 / dt
-  RET((dense_contract_vbias<CPV, GT, LVT>(c, Gp, Lv, c->e_v, "energy_v_gemm")));
+  if constexpr (CPV == CP_BF16X6) {
+    // three pieces of the split G (built for the exchange) and of Lt
+    RET((dense_contract_vbias<CP_BF16X6, bfs, bfs>(c, c->G3, Lv, c->e_v, "energy_v_gemm", c->G3l, c->tp_Lt,
+                                                   "energy_v_reduce")));
+  } else {
+    RET((dense_contract_vbias<CPV, GT, LVT>(c, Gp, Lv, c->e_v, "energy_v_gemm", nullptr, nullptr,
+                                            "energy_v_reduce")));
+  }
   RET(launch(c, "energy_direct", [&]() {
     hipLaunchKernelGGL((sumsq_kernel<zd>), dim3(nw), dim3(256), 0, c->stream, K, c->e_v, 1L, (long)K, c->e_direct,
                        inv_dt);
@@ -1702,7 +1711,10 @@ int do_energy(afqmc_ctx* c, double* eav_out)
         hipLaunchKernelGGL((split3_kernel<zd>), dim3(nblk(2L * N * M * nw, 256)), dim3(256), 0, c->stream,
                            (const zd*)c->Gtmp, c->G3, c->G3l, 2L * N * M * nw);
       }));
-      RET((energy_cholesky<CP_F64, double, CP_BF16X6, double, double, float, bfs>(c, G, L64, L64)));
+      if (c->coulomb3 && c->tp_Lt)
+        RET((energy_cholesky<CP_BF16X6, bfs, CP_BF16X6, double, double, float, bfs>(c, G, c->Lt, L64)));
+      else
+        RET((energy_cholesky<CP_F64, double, CP_BF16X6, double, double, float, bfs>(c, G, L64, L64)));
     } else if (c->energy_level == 2)
       RET((energy_cholesky<CP_F64, double, CP_F32, double, double, float, float>(c, G, L64, L64)));
     else if (c->bsplit)
@@ -1996,6 +2008,13 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
   if (option == AFQMC_OPT_CSR_DENSE) {
     if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
     c->csr_dense = value != 0.0;
+    return AFQMC_OK;
+  }
+  if (option == AFQMC_OPT_COULOMB_THREE_PIECE) {
+    if (c->have_system) return fail(c, AFQMC_ERR_STATE, "set options before uploading the system");
+    if (value != 0.0 && c->mode != AFQMC_MIXED_BF16)
+      return fail(c, AFQMC_ERR_UNSUPPORTED, "the three-piece Coulomb term is a split-BF16 option");
+    c->coulomb3 = value != 0.0;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_TRANSPOSED) {
@@ -2346,7 +2365,7 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
                              2 * K * NM);
         }
       }));
-      if (c->three & 1) {
+      if ((c->three & 1) || c->coulomb3) {
         RET(dalloc(c, &c->tp_Lt, 2 * K * NM));
         RET(launch(c, "third_pieces", [&]() {
           hipLaunchKernelGGL(third_piece_kernel, dim3(nblk(2 * K * NM, 256)), dim3(256), 0, st, Lt64, (const zb*)Lt32,

@@ -45,7 +45,7 @@ const TimerSites kTimers[] = {
      {"ortho_gram", "ortho_chol_inv", "ortho_sign", "ortho_apply", "ortho_flags", "lq_orthogonalize", "lq_fallback"}},
     {"Local Energy",
      {"energy_one", "energy_two", "energy_direct", "energy_exch_dot", "energy_pairsum", "energy_assemble",
-      "energy_zero", "energy_trace", "energy_v_gemm", "energy_vx_gemm", "energy_y_gemm", "csrmm_vakbl", "g_transpose",
+      "energy_zero", "energy_trace", "energy_v_gemm", "energy_v_reduce", "energy_vx_gemm", "energy_y_gemm", "csrmm_vakbl", "g_transpose",
       "gc64_gemm", "convert", "g_split3"}},
 };
 

@@ -38,6 +38,7 @@ def test_reference_timers_cover_every_call_site():
     names = set(re.findall(r'launch\(c, "([a-z_0-9]+)"', src))
     names |= set(re.findall(r'\(c, "([a-z_0-9]+)", a', src))
     names |= set(re.findall(r'csrmm_n\(c, "([a-z_0-9]+)"', src))
+    names |= set(re.findall(r'"([a-z_0-9]+_reduce)"', src))  # chunk sums named by their caller
     setup = {"build_vx", "vx_lower", "vx_split3", "halfrotate_gemm", "rot_count", "rot_fill", "csr_densify",
              "third_pieces", "init_walkers", "wdata_logs", "zgemm"}
     timed = [k for _, sites in m.REFERENCE_TIMERS for k in sites]

@@ -3,6 +3,7 @@
 FP64 bar: 1e-10 relative (per element, relative to the array's max
 magnitude) for every phase; the multi-step trajectory compares at 1e-9.
 """
+import ctypes as C
 import os
 import sys
 
@@ -44,6 +45,32 @@ def perturbed_walkers(s, nw, seed=3, noise=0.1):
     W[:, 1] = s.trial_beta.conj()
     W += noise * (r.standard_normal(W.shape) + 1j * r.standard_normal(W.shape))
     return W
+
+
+@pytest.mark.parametrize("M,N,nw", [(512, 128, 64), (200, 64, 32), (40, 16, 16)])
+def test_mixed_density_matrix_row_swaps(require_gpu, M, N, nw):
+    """Walkers unrelated to the trial (complex Gaussian entries): partial
+    pivoting swaps rows at most steps of the batched inverse, many of them
+    rows held by different waves (the case where the swapped-in row must
+    read old row k's column entry from this step's copy).  G and the
+    overlaps against the oracle at 1e-10."""
+    say("mdm with row swaps", M, N, nw)
+    s = m.make_system(M, N, 8, density=1.0, sigma_L=0.01, sigma_h=1.0, dt=0.005, seed=17 + M)
+    r = np.random.default_rng(M + N)
+    W = r.standard_normal((nw, 2, M, N)) + 1j * r.standard_normal((nw, 2, M, N))
+    Wd = np.zeros((nw, 8), complex)
+    g = m.AfqmcSys(s, nw, "fp64_dense")
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_WDATA, Wd)
+    G = g.calculate_mixed_density_matrix()
+    ov = g.W_data[:, 2:4].copy()
+    g.close()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wd)
+    # per walker: the overlaps of unrelated walkers span many decades
+    eg = (np.abs(G - Go).max(axis=0) / np.abs(Go).max(axis=0)).max()
+    eo = (np.abs(ov - Wd[:, 2:4]) / np.abs(Wd[:, 2:4])).max()
+    say(f"  G per-walker rel err {eg:.2e}, overlaps {eo:.2e}")
+    assert eg < 1e-10 and eo < 1e-10
 
 
 def test_halfrotate(c1):
@@ -262,6 +289,52 @@ def test_energy_terms_sum_and_one_body(c1):
         e1 = np.einsum("iw,i->w", G, s.haj.reshape(-1))
         close(t[:, 0], e1, 1e-12, f"{mode}: one-body term")
         g.close()
+
+
+def test_coulomb_three_piece(c1):
+    """AFQMC_OPT_COULOMB_THREE_PIECE: at energy level 2 of the split-BF16
+    mode the Coulomb term's contraction v = Lt.G runs in three-piece
+    split-BF16 (24-bit operands).  On identical walkers the one-body and
+    exchange terms are the FP64-Coulomb run's bit for bit, and the Coulomb
+    term stays within 3e-7 relative (per walker) of the FP64 dense path's
+    (measured 9e-8 here, |E_J| about 4 Ha: the 24-bit operands' rounding,
+    amplified by the cancellation inside each v_n).
+    The option is refused outside the split-BF16 mode and after the upload."""
+    s, p = c1
+    nw = p["nwalk"]
+    W = perturbed_walkers(s, nw, seed=21)
+    wd = np.zeros((nw, 8), complex)
+    wd[:, 1] = 1
+    terms = {}
+    for key, mode, c3 in (("fp64", "fp64_dense", False), ("c64", "mixed_bf16", False), ("c3", "mixed_bf16", True)):
+        g = m.AfqmcSys(s, nw, mode, energy_precision=2, coulomb_three_piece=c3)
+        g.set(nat.BUF_W, W)
+        g.set(nat.BUF_WDATA, wd)
+        g.calculate_mixed_density_matrix()
+        g.profile(True)
+        g.calculate_energy()
+        sites = set(g.kernel_stats())
+        g.profile(False)
+        terms[key] = g.energy_terms()
+        g.close()
+        say(f"{key}: call sites {sorted(x for x in sites if x.startswith('energy_v'))}")
+        if mode == "mixed_bf16":
+            assert "energy_v_gemm" in sites
+    np.testing.assert_array_equal(terms["c3"][:, 0], terms["c64"][:, 0])
+    np.testing.assert_array_equal(terms["c3"][:, 2], terms["c64"][:, 2])
+    e64 = np.abs(terms["c64"][:, 1] - terms["fp64"][:, 1]) / np.abs(terms["fp64"][:, 1])
+    e3 = np.abs(terms["c3"][:, 1] - terms["fp64"][:, 1]) / np.abs(terms["fp64"][:, 1])
+    say(f"Coulomb term per-walker rel err: FP64 contraction {e64.max():.2e}, three pieces {e3.max():.2e} "
+        f"(|E_J| ~ {np.abs(terms['fp64'][:, 1]).mean():.3f} Ha)")
+    assert e64.max() < 1e-12 and e3.max() < 3e-7
+    code = {v: k for k, v in nat.STATUS.items()}
+    h = C.c_void_p()
+    nat.check(nat.lib.afqmc_ctx_create(s.nmo, s.naea, s.nchol, nw, nat.MODES["fp64_dense"], C.byref(h)))
+    assert nat.lib.afqmc_set_option(h, nat.OPT_COULOMB_THREE_PIECE, 1.0) == code["AFQMC_ERR_UNSUPPORTED"]
+    nat.lib.afqmc_ctx_destroy(h)
+    g = m.AfqmcSys(s, nw, "mixed_bf16", energy_precision=2)
+    assert nat.lib.afqmc_set_option(g.ctx, nat.OPT_COULOMB_THREE_PIECE, 1.0) == code["AFQMC_ERR_STATE"]
+    g.close()
 
 
 @pytest.mark.parametrize("mode", ["mixed_fp32", "mixed_bf16"])
