@@ -155,6 +155,7 @@ struct afqmc_ctx {
   uint32_t* tp_X = nullptr;    // [nw][K] (HS potential)
   uint32_t* tp_P = nullptr;    // [M][M] (propagator)
   uint32_t* tp_vHS = nullptr;  // [nw][M*M] (propagator)
+  bool vhs3 = false;           // tp_vHS holds the third pieces of the current vHS
   uint32_t* tp_Ta = nullptr;   // [nw][2][N][M] (propagator terms)
   uint32_t* tp_Tb = nullptr;
   // with bit 4 of `three`: which of the propagator's eight products take
@@ -1049,7 +1050,10 @@ int do_vhs(afqmc_ctx* c)
   // split-BF16: Ld stored [M*M][K]
   a.sbn = K;
   a.sbk = 1;
-  a.C3 = c->tp_vHS;  // three-piece propagator: vHS with its third pieces
+  // vHS's third pieces, only when a Horner step of the three-piece
+  // propagator reads them (bits 1-6 of the step mask)
+  c->vhs3 = c->tp_vHS && (c->three_steps & 0x7E);
+  a.C3 = c->vhs3 ? c->tp_vHS : nullptr;
   if (c->three & 2) {
     a.A3 = c->tp_X;
     a.B3 = c->tp_Ld;
@@ -2512,7 +2516,13 @@ int afqmc_copy(afqmc_ctx* c, int buffer, int to_device, double* host)
   case AFQMC_BUF_VBIAS: return d ? wlast(c->vbias, K, false) : raw(c->vbias, sizeof(zd) * K * nw);
   case AFQMC_BUF_X: return d ? wlast(c->X, K, c->mixed, c->tp_X) : raw(c->X, sizeof(zd) * K * nw);
   case AFQMC_BUF_VHS:
-    if (d) return wlast(c->vHS, MM, c->mixed, c->tp_vHS);
+    if (d) {
+      // an upload writes the third pieces; a read adds them only when the
+      // last HS potential wrote them
+      RET(wlast(c->vHS, MM, c->mixed, (to_device || c->vhs3) ? c->tp_vHS : nullptr));
+      if (to_device) c->vhs3 = c->tp_vHS != nullptr;
+      return AFQMC_OK;
+    }
     RET(raw(c->vHS, sizeof(zd) * MM * nw));
     if (to_device) {
       dim3 grid((nw + 31) / 32, (unsigned)((MM + 31) / 32));
