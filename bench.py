@@ -47,7 +47,7 @@ MODE_DTYPE = {"fp64_csr": "c128", "fp64_dense": "c128", "mixed_fp32": "c64+c128"
 MEASURED_TRAFFIC = {
     # Horner steps, average of the six: 4.02 GiB read (V, U, S) + 1.09 GiB
     # written (U', and U1's third pieces) per launch
-    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.483e9,
+    ("prop_taylor", "N128_M512", "mixed_bf16"): 5.480e9,
 }
 
 
