@@ -136,8 +136,11 @@ __global__ __launch_bounds__(64 * WM * WN, WPE) void zgemm_b16dma_kernel(ZGemmAr
   // third pieces: unit u is slot u % 8 of LDS row u / 8 = 16q + r, which
   // holds chunk x / 2 (four complex) of tile row 32q + 16 (x & 1) + r, where
   // x = (u % 8) ^ swz(r)
-  auto row3 = [](int u) { const int sr = u >> 3, x = (u & 7) ^ b16::swz(sr & 15); return 32 * (sr >> 4) + 16 * (x & 1) + (sr & 15); };
-  auto chunk3 = [](int u) { return (((u & 7) ^ b16::swz((u >> 3) & 15)) >> 1); };
+  auto row3 = [](int u) {
+    const int sr = u >> 3, x = (u & 7) ^ b16::swz(sr & 15);
+    return 32 * (sr >> 4) + 16 * (x & 1) + (sr & 15);
+  };
+  auto chunk3 = [](int u) { return ((u & 7) ^ b16::swz((u >> 3) & 15)) >> 1; };
   const uint32_t* pa3[AL3 > 0 ? AL3 : 1];
   const uint32_t* pb3[BL3 > 0 ? BL3 : 1];
 #pragma unroll
