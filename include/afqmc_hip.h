@@ -141,7 +141,8 @@ typedef enum {
    * three bf16 pieces, 24 significant bits, six products accumulated in
    * FP32) instead of two pieces: 1 the bias product (G from an FP32 product),
    * 2 the HS potential, 4 the propagator (P W, the Taylor terms, P U).
-   * Needs M, K and N*M multiples of 16.  Default 0.  No counterpart in the
+   * Needs M, K and N*M multiples of 16 for the bits that contract over them
+   * (AFQMC_ERR_UNSUPPORTED otherwise).  Default 0.  No counterpart in the
    * reference. */
   AFQMC_OPT_THREE_PIECE = 8,
   /* with bit 4 of AFQMC_OPT_THREE_PIECE: a mask of the propagator's eight

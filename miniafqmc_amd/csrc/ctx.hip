@@ -2044,7 +2044,14 @@ int afqmc_set_option(afqmc_ctx* c, int option, double value)
     if (value < 0 || value > 7 || value != (int)value) return fail(c, AFQMC_ERR_ARG, "three-piece mask is 0..7");
     if (value != 0 && c->mode != AFQMC_MIXED_BF16)
       return fail(c, AFQMC_ERR_UNSUPPORTED, "three-piece contractions are a split-BF16 option");
-    c->three = (int)value;
+    // three-piece products run on the staged-ring kernel only (k-tiles of
+    // 16 complex): the propagator contracts over M, the HS potential over K,
+    // the bias product over N*M
+    const int v = (int)value;
+    if (((v & 4) && c->M % 16) || ((v & 2) && c->K % 16) || ((v & 1) && ((long)c->N * c->M) % 16))
+      return fail(c, AFQMC_ERR_UNSUPPORTED,
+                  "three-piece contractions need their contraction length (M, K, N*M) a multiple of 16");
+    c->three = v;
     return AFQMC_OK;
   }
   if (option == AFQMC_OPT_THREE_PIECE_STEPS) {
@@ -2348,7 +2355,10 @@ int afqmc_upload_system(afqmc_ctx* c, const double* trial_alpha, const double* t
     dfree(c, val);
     if (!c->mixed || c->energy_level >= 3) RET(maybe_vx<zd>(c, Lt64, sizeof(zd)));
     else if (c->bsplit && c->energy_level <= 1) RET(maybe_vx<zb>(c, Lt64, sizeof(zf)));
-    else if (c->bsplit) RET(maybe_vx3(c, Lt64));
+    // the three-piece exchange runs on the staged-ring kernel, whose k-tiles
+    // are 16 complex (and chunk offsets 16-byte aligned): other sizes keep
+    // the FP32 block, as the FP32 mode's level 2
+    else if (c->bsplit && NM % 16 == 0) RET(maybe_vx3(c, Lt64));
     else RET(maybe_vx<zf>(c, Lt64, sizeof(zf)));
     if (c->mixed) {
       zf* Ld32 = nullptr;

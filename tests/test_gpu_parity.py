@@ -586,6 +586,39 @@ def test_run_ragged_shapes(require_gpu, M, N, K, nw, mode):
         close(g["Wd"][:, :5], o["Wd"][:, :5], 1e-9, "final W_data")
 
 
+def test_ragged_split_bf16_energy_level2(require_gpu):
+    """Energy level 2 of the split-BF16 mode at a shape whose N*M (280) is
+    no multiple of 16: the exchange takes the FP32 block (the three-piece
+    one needs 16-complex k-tiles) and E_L stays within 1e-5 Ha of the
+    oracle; the three-piece contraction options are refused at such shapes
+    (M = 40 for the propagator, K = 53 for the HS potential, N*M = 280 for
+    the bias)."""
+    M, N, K, nw = 40, 7, 53, 5
+    say("ragged split-BF16 energy level 2", M, N, K, nw)
+    s = m.make_system(M, N, K, density=0.5, sigma_L=0.03, sigma_h=1.0, dt=0.005, seed=M * 100 + N, with_vakbl=True)
+    W = perturbed_walkers(s, nw, seed=5)
+    wd = np.zeros((nw, 8), complex)
+    wd[:, 1] = 1
+    Wdo = wd.copy()
+    Go, _ = orc.calculate_mixed_density_matrix(s, W, Wdo)
+    orc.calculate_energy(s, Wdo, Go)
+    g = m.AfqmcSys(s, nw, "mixed_bf16", energy_precision=2)
+    g.set(nat.BUF_W, W)
+    g.set(nat.BUF_WDATA, wd)
+    g.calculate_mixed_density_matrix()
+    g.calculate_energy()
+    dE = np.abs(g.W_data[:, 0] - Wdo[:, 0])
+    say(f"  max |dE_L| = {dE.max():.3e} Ha (E_L ~ {Wdo[:, 0].real.mean():.3f} Ha)")
+    assert dE.max() < 1e-5
+    g.close()
+    code = {v: k for k, v in nat.STATUS.items()}
+    h = C.c_void_p()
+    nat.check(nat.lib.afqmc_ctx_create(M, N, K, nw, nat.MODES["mixed_bf16"], C.byref(h)))
+    for mask in (4, 2, 1):
+        assert nat.lib.afqmc_set_option(h, nat.OPT_THREE_PIECE, float(mask)) == code["AFQMC_ERR_UNSUPPORTED"]
+    nat.lib.afqmc_ctx_destroy(h)
+
+
 @pytest.mark.parametrize("bad", ["column", "rowptr"])
 def test_upload_rejects_malformed_spvn(require_gpu, bad):
     """A Cholesky CSR with an out-of-range column or a decreasing row pointer
