@@ -129,6 +129,20 @@ def kernel_flops_total(name, launches, M, N, K, nw, n_energy, mode=None):
     return None if f is None else f * launches
 
 
+def roofline_csr(csr, args, M, N, K, nw, nnz_spvn, nnz_spvnT):
+    """Roofline entry of the FP64 CSR mode's sparse product with the most
+    time (`csr`: (call site, (launches, ms)) pairs): HBM bandwidth against
+    its algorithmic bytes, beside the FP64 rate it reaches."""
+    name, (nl, ms) = max(csr, key=lambda kv: kv[1][1])
+    avg_s = ms / nl / 1e3
+    nbytes = kernel_bytes(name, M, N, K, nw, nnz_spvn, nnz_spvnT)
+    fl = csr_flops(name, nw, nnz_spvn, nnz_spvnT)
+    ach = nbytes / avg_s / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
+            "avg_ms": round(ms / nl, 4), "bytes_per_launch": nbytes, "fp64_tflops": round(fl / avg_s / 1e12, 2)}
+
+
 def roofline_rows(stats, args, M, N, K, nw, n_energy):
     """Roofline rows of the call sites with a known algorithmic count, most
     total time first, and the bench line's `roofline` entry for the first.
@@ -478,17 +492,7 @@ def main():
     n_energy = 0 if args.no_energy else args.steps // spe
     csr = [(k_, v) for k_, v in stats.items() if k_ in ("csrmm_vhs", "csrmm_vbias")]
     if args.mode == "fp64_csr" and csr:
-        # the reference's sparse products: report HBM bandwidth against the
-        # algorithmic bytes, beside the FP64 rate they reach
-        name, (nl, ms) = max(csr, key=lambda kv: kv[1][1])
-        avg_s = ms / nl / 1e3
-        nbytes = kernel_bytes(name, M, N, K, nw, s.spvn.nnz, res["nnz_spvnT"])
-        fl = csr_flops(name, nw, s.spvn.nnz, res["nnz_spvnT"])
-        ach = nbytes / avg_s / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": MEASURED_TRAFFIC.get((name, args.workload, args.mode)),
-                "avg_ms": round(ms / nl, 4), "bytes_per_launch": nbytes,
-                "fp64_tflops": round(fl / avg_s / 1e12, 2)}
+        roof = roofline_csr(csr, args, M, N, K, nw, s.spvn.nnz, res["nnz_spvnT"])
     else:
         roof, rows = roofline_rows(stats, args, M, N, K, nw, n_energy)
         for r in rows:

@@ -85,3 +85,23 @@ def test_roofline_bound_follows_intensity():
     args.mode = "mixed_fp32"
     roof32, _ = bench.roofline_rows(stats, args, M, N, K, nw, n_energy=1)
     assert roof32["bound"] == "mfma"
+
+
+def test_roofline_csr_sparse_product():
+    """FP64 CSR mode: the sparse product with the most time is reported
+    against HBM with its algorithmic bytes (16 B value + 4 B column per
+    entry, the row pointers, the dense operand read once and the output
+    written once) and its FP64 rate (one complex multiply-add per entry
+    and walker)."""
+    import argparse
+    sys.path.insert(0, ROOT)
+    import bench
+    M, N, K, nw = 256, 64, 1024, 256
+    nnz, nnzT = 13_400_000, 33_500_000
+    args = argparse.Namespace(mode="fp64_csr", workload="N64_M256")
+    csr = [("csrmm_vhs", (20, 20 * 2.355)), ("csrmm_vbias", (20, 20 * 1.0))]
+    roof = bench.roofline_csr(csr, args, M, N, K, nw, nnz, nnzT)
+    nbytes = nnz * 20 + (M * M + 1) * 4 + K * nw * 16 + M * M * nw * 16
+    assert roof["kernel"] == "csrmm_vhs" and roof["bound"] == "hbm" and roof["bytes_per_launch"] == nbytes
+    assert abs(roof["achieved"] - nbytes / 2.355e-3 / 1e9) < 0.1
+    assert abs(roof["fp64_tflops"] - 8.0 * nnz * nw / 2.355e-3 / 1e12) < 0.01
